@@ -1,0 +1,185 @@
+"""Benchmark: Mrays/s of the GPU path tracer on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): a displaced
+icosphere of 1,003,520 triangles (frequency-224 geodesic, 3-octave value
+noise amplitude 0.05, area-weighted smooth normals, 70% matte / 30% metal by
+face hash), two r = 0.1 lights of radiance 50, 1920x1080, 64 spp, 10
+bounces, Russian roulette from bounce 3, NEE on, FP32. One step = one full
+frame. Rays are counted as the reference counts them: one per closest-hit
+query and one per light sample that reaches the occlusion test
+(renderer.hpp:19,72,110).
+
+Besides the timed steps it reports:
+  roofline      the closest-hit traversal stage (the dominant kernel): its
+                algorithmic bytes (per query 32 in + 16 out, 128 per node,
+                4 per leaf reference, 48 per triangle, 16 per sphere, 52 per
+                instance, 16 per large sphere; visit counts from a counting
+                render of the same rays) over its HIP-event time per launch;
+  cpu_baseline  the reference's own renderer (oracle/_ref, its headers
+                compiled with its flags) on the host cores, on a bounded
+                sample of the same frame.
+
+python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s (1 GPU); equal-time relMSE guided vs plain NEE, 1080p 1M tris"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ptsd, width, height, spp_full, seconds):
+    """The reference renderer on the host cores over a bounded sample of the
+    frame: every row at a reduced spp, sized to about `seconds` of work."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if ob.ref_available():
+        world = ob.RefWorld(ptsd)
+        kind = "reference"
+
+        def run(spp):
+            _, _, rays, secs = world.render(width, height, spp, seed=1, threads=threads)
+            return rays, secs
+    else:
+        import photon_tracer_amd as pt
+
+        ptgp = pt.pack(ptsd)
+        world = None
+        kind = "port"
+
+        def run(spp):
+            t0 = time.perf_counter()
+            _, _, rays = ob.or_render(ptgp, width, height, spp, seed=1, threads=threads)
+            return rays, time.perf_counter() - t0
+    rays, secs = run(1)
+    spp = max(1, min(spp_full, int(seconds / max(secs, 1e-3))))
+    if spp > 1:
+        rays, secs = run(spp)
+    if world is not None:
+        world.close()
+    return {
+        "value": round(rays / secs / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"full {width}x{height} frame at {spp} spp (of {spp_full}), seed 1, {rays} rays in {secs:.2f} s, "
+                  f"rows claimed by {threads} threads from an atomic counter as the reference CLI does",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--scale", type=float, default=1.0, help="triangle-count scale (1.0 = 1,003,520)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "the path shards by samples or image tiles, see the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import photon_tracer_amd as pt
+
+    t0 = time.perf_counter()
+    ptsd = pt.generate(1, 1, args.scale)
+    t1 = time.perf_counter()
+    ptgp = pt.pack(ptsd)
+    t2 = time.perf_counter()
+    n_tris = int(ptsd[13])
+    log(f"scene: {n_tris} triangles, generated in {t1 - t0:.2f} s, BVH built and packed in {t2 - t1:.2f} s "
+        f"(depth {pt.ptgp_depth(ptgp)}), {ptgp.size * 4 / 1e6:.1f} MB")
+    W, H, spp = args.width, args.height, args.spp
+    scene = pt.Scene(ptgp)
+    arch, cus = pt.device_info()
+    log(f"device: {arch}, {cus} CUs")
+
+    for k in range(args.warmup):
+        _, _, st = scene.render(W, H, spp, seed=1000 + k)
+        log(f"warmup {k}: {st.render_ms:.1f} ms, {st.mrays_per_s:.0f} Mrays/s")
+
+    rays = 0
+    gpu_ms = 0.0
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        _, _, st = scene.render(W, H, spp, seed=1 + k)
+        rays += st.rays
+        gpu_ms += st.render_ms
+        log(f"step {k}: {st.rays} rays ({st.primary_rays} primary, {st.shadow_rays} shadow), "
+            f"{st.render_ms:.1f} ms on the device")
+    wall = time.perf_counter() - t_start
+    value = rays / wall / 1e6
+    samples = W * H * spp * args.steps
+    log(f"timed: {args.steps} frames in {wall:.3f} s wall ({gpu_ms:.1f} ms device): {value:.1f} Mrays/s, "
+        f"{samples / wall / 1e6:.1f} Msamples/s")
+
+    roofline = None
+    if not args.no_roofline:
+        # The same rays as step 0 (seed 1): one render counting the closest-hit
+        # stage's visits, one timing each of its launches with HIP events.
+        _, _, cst = scene.render(W, H, spp, seed=1, count=True)
+        _, _, tst = scene.render(W, H, spp, seed=1, timing=True)
+        queries = cst.rays - cst.shadow_rays
+        algo = pt.QUERY_BYTES * queries + sum(v * b for v, b in zip(cst.visits, pt.VISIT_BYTES))
+        launches = 10 * (spp // max(1, spp_per_batch(W, H, spp)))
+        achieved = algo / (tst.extend_ms * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_extend<16, false> (closest-hit traversal stage)",
+            "launches": launches, "avg_launch_ms": round(tst.extend_ms / launches, 4),
+            "algorithmic_bytes_per_launch": int(algo / launches),
+            "per_query": {"queries": queries, "nodes": cst.visits[0] / queries, "leafrefs": cst.visits[1] / queries,
+                          "tris": cst.visits[2] / queries, "spheres": cst.visits[3] / queries},
+        }
+        log(f"roofline: {json.dumps(roofline)}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ptsd, W, H, spp, args.cpu_seconds)
+        log(f"cpu baseline: {json.dumps(cpu)}")
+    scene.close()
+
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": f"displaced icosphere, {n_tris} triangles, {W}x{H}, {spp} spp, 10 bounces, NEE, "
+                               "RR from bounce 3", "width": W, "height": H, "spp": spp, "triangles": n_tris,
+                   "parallelism": "single GPU"},
+        "samples_per_s": round(samples / wall, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+def spp_per_batch(width, height, spp):
+    """The library's own choice (api.hip choose_spb): the largest divisor of
+    spp with at most 2^23 paths in flight."""
+    cap = max(1, (1 << 23) // (width * height))
+    return max(d for d in range(1, spp + 1) if spp % d == 0 and d <= cap)
+
+
+if __name__ == "__main__":
+    main()

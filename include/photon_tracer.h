@@ -1,0 +1,186 @@
+/*
+ * photon_tracer.h -- C ABI of the MI355X path-tracing core.
+ *
+ * The core is a drop-in for the render path of the reference CPU path tracer
+ * (apollo-2006/photon_tracer). Every entry point below replaces one call of
+ * that path; the comment on each names the reference interface it stands in
+ * for (file:line in the reference tree).
+ *
+ * Conventions
+ *   - Plain C: pointers and sizes, no C++ types, no exceptions across the ABI.
+ *   - Every function returns a pt_status; on failure pt_last_error() gives the
+ *     message (thread-local), where the reference throws std::runtime_error
+ *     (gpu_pack.hpp:62,86) and its CLI returns 1 (main.cpp:103-108).
+ *   - The caller owns host arrays. A pt_scene owns its device buffers, freed by
+ *     pt_scene_destroy. Output goes to caller memory.
+ *   - One HIP stream per scene handle: calls on one handle from several host
+ *     threads must be serialised (the reference is reentrant per thread over a
+ *     const geometry, renderer.hpp:17-19).
+ *
+ * Data formats
+ *   PTSD ("scene description") is this library's own word buffer holding the
+ *   reference `geometry` flat arrays (geometry.hpp:15-58) and the camera
+ *   parameters (camera.hpp:18-34). Layout in pt_layout comments below.
+ *   PTGP v1 is the reference's packed GPU scene, byte for byte
+ *   (gpu_pack.hpp:21-33,77-82,177-203): 48-word header, 8 sections, 16-byte
+ *   aligned. pt_pack produces it from PTSD with this library's own binned-SAH
+ *   4-wide builder, which emits the reference's node layout (bvh.hpp:27-32),
+ *   without the reference's 10-level depth cap (gpu_pack.hpp:36,61-62).
+ */
+#ifndef PHOTON_TRACER_H
+#define PHOTON_TRACER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PT_OK = 0,
+    PT_ERR_INVALID = 1,   /* bad argument or malformed buffer */
+    PT_ERR_DEVICE = 2,    /* a HIP call failed */
+    PT_ERR_NOMEM = 3,     /* host or device allocation failed */
+    PT_ERR_LIMIT = 4      /* a size beyond what the library supports */
+} pt_status;
+
+typedef struct pt_scene pt_scene;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* pt_last_error(void);
+
+/* Library version string and the device it runs on (name, CU count). */
+const char* pt_version(void);
+pt_status pt_device_info(char* name, size_t name_len, int* compute_units);
+
+/* Frees a buffer returned by pt_generate / pt_pack. */
+void pt_free(void* p);
+
+/* ---------------------------------------------------------------------------
+ * Scene description and packing (host side).
+ * ------------------------------------------------------------------------- */
+
+/* Seeded procedural scenes of the benchmark configurations.
+ *   config 0: sphere field, 1: displaced icosphere mesh, 2: instanced meshes,
+ *   4: closed room with a displaced mesh and lights behind a partition.
+ *   (3, the batch, is a list of config-0/1 scenes: see pt_generate_job.)
+ * `scale` shrinks the triangle counts for tests (1.0 = the full sizes).
+ * Replaces the reference's scene builders (renderer.hpp:158-343) for scenes
+ * the reference does not have. Returns a PTSD buffer (free with pt_free). */
+pt_status pt_generate(int config, uint64_t seed, double scale, uint32_t** ptsd, size_t* n_words);
+
+/* One job of the batched workload: a config-0 or config-1 type scene with a
+ * seeded primitive count, and its resolution and spp. */
+typedef struct {
+    int32_t kind;          /* 0 spheres, 1 triangle mesh */
+    int32_t width, height;
+    int32_t spp;
+    uint32_t primitives;   /* sphere or triangle count */
+    uint64_t seed;
+} pt_job_desc;
+pt_status pt_generate_job(uint64_t seed, double scale, pt_job_desc* desc, uint32_t** ptsd, size_t* n_words);
+
+/* Builds every BVH of a PTSD scene (binned SAH, 16 bins, max leaf 8, 4-wide
+ * collapse; bvh.hpp:39-52,116-247; geometry.hpp:74-104) and packs the scene
+ * as PTGP v1 (pack_for_gpu + serialize, gpu_pack.hpp:84-203). */
+pt_status pt_pack(const uint32_t* ptsd, size_t n_words, uint32_t** ptgp, size_t* ptgp_words);
+
+/* Depth (levels of 4-wide nodes) of the deepest tree in a PTGP buffer. */
+pt_status pt_ptgp_depth(const uint32_t* ptgp, size_t n_words, int* depth);
+
+/* ---------------------------------------------------------------------------
+ * Device scene.
+ * ------------------------------------------------------------------------- */
+
+/* Copies a PTGP v1 buffer to the device. Replaces the page's GpuRenderer
+ * loadScene (web/gpu.js:59-86), which uploads serialize(pack_for_gpu(...)).
+ * Accepts trees deeper than the reference's WebGPU stack allows. */
+pt_status pt_scene_create(const uint32_t* ptgp, size_t n_words, pt_scene** scene);
+pt_status pt_scene_destroy(pt_scene* scene);
+
+/* ---------------------------------------------------------------------------
+ * Ray queries: the closest hit of geometry::hit (geometry.hpp:111-152) and
+ * the any hit of geometry::occluded (geometry.hpp:156-190), over a batch.
+ * ------------------------------------------------------------------------- */
+
+/* A ray: origin, t_min, direction, t_max (32 bytes). */
+typedef struct {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} pt_ray;
+
+/* kind values of pt_hit */
+enum { PT_MISS = 0, PT_LARGE_SPHERE = 1, PT_SPHERE = 2, PT_TRIANGLE = 3 };
+
+/* A hit: t, barycentrics, what was hit. `prim` is the PTGP sphere number
+ * (kinds 1-2) or PTGP triangle number (kind 3); `inst` is the instance number
+ * for a triangle reached through the instance tree, else 0xffffffff. For any
+ * hit queries only `kind` (miss or not) is meaningful. */
+typedef struct {
+    float t, u, v;
+    uint32_t kind;
+    uint32_t prim;
+    uint32_t inst;
+} pt_hit;
+
+/* Host arrays in and out. any_hit = 0: closest hit; 1: any hit. */
+pt_status pt_trace(pt_scene* scene, const pt_ray* rays, size_t n, int any_hit, pt_hit* hits);
+
+/* Device arrays in and out (rays and hits already in HBM), on the scene's
+ * stream; `elapsed_ms` (optional) gets the traversal kernel's time from HIP
+ * events on that stream. `visits` (optional, 6 entries) runs the counting
+ * variant of the kernel and gets the batch's totals of node slab tests, leaf
+ * references, triangle tests, sphere tests, instance entries and large-sphere
+ * tests, which price its algorithmic bytes. */
+pt_status pt_trace_device(pt_scene* scene, const void* d_rays, size_t n, int any_hit, void* d_hits,
+                          float* elapsed_ms, uint64_t* visits);
+
+/* ---------------------------------------------------------------------------
+ * Rendering: render_row_linear (renderer.hpp:411-416) for a whole frame, or a
+ * band of rows, as a wavefront path loop on the GPU.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+    int32_t width, height;    /* image size (the camera is the PTGP one) */
+    int32_t spp;              /* samples per pixel */
+    int32_t max_bounces;      /* 10 in the reference CLI (main.cpp:84) */
+    uint64_t seed;
+    int32_t first_sample;     /* progressive continuation (renderer.hpp:345-356) */
+    int32_t row_begin;        /* rows [row_begin, row_end) counted from the   */
+    int32_t row_end;          /* bottom (j), as render_row_linear; 0,0 = all  */
+    int32_t flags;            /* PT_RENDER_* */
+    int32_t samples_per_batch;/* paths in flight per pixel per batch; 0 = auto */
+    int32_t stack_entries;    /* LDS short-stack entries per lane; 0 = auto   */
+} pt_render_params;
+
+enum {
+    PT_RENDER_NO_NEE = 1,        /* world.sample_lights = false (main.cpp --no-nee) */
+    PT_RENDER_NO_GRAPH = 2,      /* launch the bounce loop eagerly, not as a hipGraph */
+    PT_RENDER_TIMING = 4,        /* eager, with HIP events around every traversal launch */
+    PT_RENDER_COUNT = 8          /* eager, closest-hit stage counts its visits (stats->visits) */
+};
+
+/* Render statistics. */
+typedef struct {
+    uint64_t rays;             /* counted as the reference: renderer.hpp:19,72,110 */
+    uint64_t primary_rays;
+    uint64_t shadow_rays;
+    double render_ms;          /* device time of the render, HIP events */
+    double extend_ms;          /* with PT_RENDER_TIMING: time in closest-hit traversal */
+    double shadow_ms;          /* with PT_RENDER_TIMING: time in any-hit traversal */
+    uint64_t visits[6];        /* with PT_RENDER_COUNT: closest-hit node, leaf-ref,
+                                  triangle, sphere, instance, large-sphere visits */
+} pt_render_stats;
+
+/* rgb: 3 floats a pixel, top row first, for the rows rendered (the band's top
+ * row first). aux: NULL, or 7 floats a pixel (albedo, normal, luminance^2;
+ * renderer.hpp:358-362). stats: optional. */
+pt_status pt_render(pt_scene* scene, const pt_render_params* params, float* rgb, float* aux,
+                    pt_render_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOTON_TRACER_H */

@@ -1,0 +1,296 @@
+"""photon_tracer_amd: an MI355X path-tracing core behind a C ABI.
+
+The native library (``libphoton_tracer_amd.so``, built in this directory from
+``csrc/``) does all the work: scene generation and packing on the host, BVH
+traversal and the wavefront path loop as gfx950 HIP kernels. This module only
+binds its C ABI (``include/photon_tracer.h``) with ctypes and moves numpy
+arrays across it. There is no fallback: if the library is missing, importing
+the bindings raises.
+
+Reference correspondence (apollo-2006/photon_tracer):
+  generate / pack   scene builders + geometry::build + pack_for_gpu
+                    (renderer.hpp:158-343, geometry.hpp:74-104, gpu_pack.hpp:84-203)
+  Scene.trace       geometry::hit / geometry::occluded (geometry.hpp:111-190)
+  Scene.render      render_row_linear over a frame or band of rows
+                    (renderer.hpp:366-416)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphoton_tracer_amd.so")
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("kind", "<u4"), ("prim", "<u4"), ("inst", "<u4")])
+MISS, LARGE_SPHERE, SPHERE, TRIANGLE = 0, 1, 2, 3
+NONE = 0xFFFFFFFF
+
+RENDER_NO_NEE = 1
+RENDER_NO_GRAPH = 2
+RENDER_TIMING = 4
+RENDER_COUNT = 8
+
+# Algorithmic bytes of one traversal query (the ray in, the hit out, and every
+# record the reference's walk must read): 32 + 16 per query, 128 per node slab
+# test, 4 per leaf reference, 48 per triangle test, 16 per sphere test, 52 per
+# instance entry (to_object rows + root), 16 per large-sphere test.
+VISIT_BYTES = (128, 4, 48, 16, 52, 16)
+QUERY_BYTES = 32 + 16
+
+
+class PTError(RuntimeError):
+    """A failure reported by the library (status code and pt_last_error())."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+
+
+class _RenderParams(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
+        ("max_bounces", ctypes.c_int32), ("seed", ctypes.c_uint64), ("first_sample", ctypes.c_int32),
+        ("row_begin", ctypes.c_int32), ("row_end", ctypes.c_int32), ("flags", ctypes.c_int32),
+        ("samples_per_batch", ctypes.c_int32), ("stack_entries", ctypes.c_int32),
+    ]
+
+
+class _RenderStats(ctypes.Structure):
+    _fields_ = [
+        ("rays", ctypes.c_uint64), ("primary_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+        ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
+        ("visits", ctypes.c_uint64 * 6),
+    ]
+
+
+class _JobDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
+        ("primitives", ctypes.c_uint32), ("seed", ctypes.c_uint64),
+    ]
+
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """The loaded native library (loaded on first use)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C photon_tracer_amd")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sz = ctypes.c_size_t
+    lib.pt_last_error.restype = ctypes.c_char_p
+    lib.pt_version.restype = ctypes.c_char_p
+    lib.pt_free.argtypes = [vp]
+    lib.pt_device_info.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_int)]
+    lib.pt_generate.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(_u32p),
+                                ctypes.POINTER(sz)]
+    lib.pt_generate_job.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(_JobDesc),
+                                    ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
+    lib.pt_pack.argtypes = [vp, sz, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
+    lib.pt_ptgp_depth.argtypes = [vp, sz, ctypes.POINTER(ctypes.c_int)]
+    lib.pt_scene_create.argtypes = [vp, sz, ctypes.POINTER(vp)]
+    lib.pt_scene_destroy.argtypes = [vp]
+    lib.pt_trace.argtypes = [vp, vp, sz, ctypes.c_int, vp]
+    lib.pt_trace_device.argtypes = [vp, vp, sz, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_float),
+                                    ctypes.POINTER(ctypes.c_uint64)]
+    lib.pt_render.argtypes = [vp, ctypes.POINTER(_RenderParams), vp, vp, ctypes.POINTER(_RenderStats)]
+    for name in ("pt_device_info", "pt_generate", "pt_generate_job", "pt_pack", "pt_ptgp_depth", "pt_scene_create",
+                 "pt_scene_destroy", "pt_trace", "pt_trace_device", "pt_render"):
+        getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise PTError(status, library().pt_last_error().decode(errors="replace"))
+
+
+def _take_words(ptr, n) -> np.ndarray:
+    try:
+        return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
+    finally:
+        library().pt_free(ptr)
+
+
+def version() -> str:
+    return library().pt_version().decode()
+
+
+def device_info() -> tuple[str, int]:
+    """(gfx arch name, compute units) of the current HIP device."""
+    name = ctypes.create_string_buffer(128)
+    cus = ctypes.c_int(0)
+    _check(library().pt_device_info(name, 128, ctypes.byref(cus)))
+    return name.value.decode(), cus.value
+
+
+def generate(config: int, seed: int = 0, scale: float = 1.0) -> np.ndarray:
+    """A benchmark scene (config 0, 1, 2 or 4) as a PTSD word array."""
+    p, n = _u32p(), ctypes.c_size_t(0)
+    _check(library().pt_generate(int(config), int(seed), float(scale), ctypes.byref(p), ctypes.byref(n)))
+    return _take_words(p, n)
+
+
+@dataclass
+class Job:
+    kind: int
+    width: int
+    height: int
+    spp: int
+    primitives: int
+    seed: int
+    ptsd: np.ndarray
+
+
+def generate_job(seed: int, scale: float = 1.0) -> Job:
+    """One job of the batched workload (config 3)."""
+    d = _JobDesc()
+    p, n = _u32p(), ctypes.c_size_t(0)
+    _check(library().pt_generate_job(int(seed), float(scale), ctypes.byref(d), ctypes.byref(p), ctypes.byref(n)))
+    return Job(d.kind, d.width, d.height, d.spp, d.primitives, d.seed, _take_words(p, n))
+
+
+def _words(a) -> np.ndarray:
+    a = np.ascontiguousarray(a)
+    if a.dtype != np.uint32:
+        a = np.frombuffer(a.tobytes(), dtype=np.uint32)
+    return a
+
+
+def pack(ptsd) -> np.ndarray:
+    """Builds the BVHs of a PTSD scene and packs it as PTGP v1 words."""
+    w = _words(ptsd)
+    p, n = _u32p(), ctypes.c_size_t(0)
+    _check(library().pt_pack(w.ctypes.data, w.size, ctypes.byref(p), ctypes.byref(n)))
+    return _take_words(p, n)
+
+
+def ptgp_depth(ptgp) -> int:
+    w = _words(ptgp)
+    d = ctypes.c_int(0)
+    _check(library().pt_ptgp_depth(w.ctypes.data, w.size, ctypes.byref(d)))
+    return d.value
+
+
+@dataclass
+class RenderStats:
+    rays: int
+    primary_rays: int
+    shadow_rays: int
+    render_ms: float
+    extend_ms: float
+    shadow_ms: float
+    visits: tuple = (0, 0, 0, 0, 0, 0)
+
+    @property
+    def mrays_per_s(self) -> float:
+        return self.rays / (self.render_ms * 1e-3) / 1e6 if self.render_ms > 0 else 0.0
+
+
+def rays_array(o, d, tmin=0.001, tmax=1000.0) -> np.ndarray:
+    """n x 8 float32 rays (o, tmin, d, tmax) from origin and direction arrays."""
+    o = np.asarray(o, np.float32).reshape(-1, 3)
+    d = np.asarray(d, np.float32).reshape(-1, 3)
+    n = max(len(o), len(d))
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+class Scene:
+    """A packed scene resident on the GPU (pt_scene)."""
+
+    def __init__(self, ptgp):
+        self._words = _words(ptgp)
+        h = ctypes.c_void_p()
+        _check(library().pt_scene_create(self._words.ctypes.data, self._words.size, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _check(library().pt_scene_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def words(self) -> np.ndarray:
+        return self._words
+
+    def trace(self, rays, any_hit: bool = False) -> np.ndarray:
+        """Closest (or any) hit of n rays given as n x 8 float32 (o, tmin, d, tmax)."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.zeros(len(r), dtype=HIT_DTYPE)
+        _check(library().pt_trace(self._h, r.ctypes.data, len(r), 1 if any_hit else 0, hits.ctypes.data))
+        return hits
+
+    def trace_device(self, d_rays: int, n: int, d_hits: int, any_hit: bool = False, count: bool = False):
+        """Traversal over device buffers (addresses). Returns (kernel ms, visit totals or None)."""
+        ms = ctypes.c_float(0)
+        vis = (ctypes.c_uint64 * 6)()
+        _check(library().pt_trace_device(self._h, ctypes.c_void_p(d_rays), n, 1 if any_hit else 0,
+                                         ctypes.c_void_p(d_hits), ctypes.byref(ms), vis if count else None))
+        return ms.value, (tuple(vis) if count else None)
+
+    def render(self, width: int, height: int, spp: int, max_bounces: int = 10, seed: int = 0,
+               first_sample: int = 0, rows: tuple[int, int] | None = None, nee: bool = True, aux: bool = False,
+               graph: bool = True, timing: bool = False, count: bool = False, samples_per_batch: int = 0,
+               stack_entries: int = 0):
+        """Renders rows [j0, j1) (from the bottom; default all). Returns (rgb, aux, stats) with rgb of shape
+        (rows, width, 3), the band's top row first, as render_row_linear writes rows."""
+        j0, j1 = rows if rows else (0, height)
+        p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, j0, j1,
+                          (0 if nee else RENDER_NO_NEE) | (0 if graph else RENDER_NO_GRAPH) |
+                          (RENDER_TIMING if timing else 0) | (RENDER_COUNT if count else 0), samples_per_batch,
+                          stack_entries)
+        nrows = j1 - j0
+        rgb = np.zeros((max(nrows, 0), width, 3), np.float32)
+        ax = np.zeros((max(nrows, 0), width, 7), np.float32) if aux else None
+        st = _RenderStats()
+        _check(library().pt_render(self._h, ctypes.byref(p), rgb.ctypes.data, ax.ctypes.data if aux else None,
+                                   ctypes.byref(st)))
+        stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
+                            tuple(st.visits))
+        return rgb, ax, stats
+
+
+def write_pfm(path: str, rgb: np.ndarray) -> None:
+    """Little-endian PFM, bottom row first, as image_write.hpp:15-28 writes it."""
+    h, w, _ = rgb.shape
+    with open(path, "wb") as f:
+        f.write(f"PF\n{w} {h}\n-1.0\n".encode())
+        f.write(np.ascontiguousarray(rgb[::-1], dtype="<f4").tobytes())
+
+
+def read_pfm(path: str) -> np.ndarray:
+    """(h, w, 3) float32, top row first."""
+    with open(path, "rb") as f:
+        assert f.readline().strip() == b"PF", "not a color PFM"
+        w, h = map(int, f.readline().split())
+        scale = float(f.readline())
+        data = np.frombuffer(f.read(), dtype="<f4" if scale < 0 else ">f4")
+    return data.reshape(h, w, 3)[::-1].astype(np.float32)

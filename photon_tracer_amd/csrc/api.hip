@@ -1,0 +1,602 @@
+// C ABI: device scenes, ray queries and rendering.
+//
+// A pt_scene holds the PTGP buffer in HBM as uploaded (the sections are used
+// in place), plus one derived array: each leaf slot's primitive data in a
+// dense 48-byte record, so a leaf's primitives are read contiguously rather
+// than through order[] into 112-byte triangle records.
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "../../include/photon_tracer.h"
+#include "kernels.hpp"
+#include "status.hpp"
+
+using ptd::NONE;
+
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw pt::status_error(e == hipErrorOutOfMemory ? PT_ERR_NOMEM : PT_ERR_DEVICE,
+                               std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+struct dbuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        HIPC(hipMalloc(&p, n ? n : 16));
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+constexpr size_t kTargetPaths = size_t(1) << 23;  // paths in flight per batch
+constexpr int kDefaultStack = 16;
+
+struct render_bufs {
+    dbuf ro[2], rd[2], rpath[2], hits, so, sd, sl, pa, pb, pkey, pq, paux;
+    dbuf accum, aux_accum, ctrl, batch, totals, out_rgb, out_aux;
+    void release() {
+        for (dbuf* b : {&ro[0], &ro[1], &rd[0], &rd[1], &rpath[0], &rpath[1], &hits, &so, &sd, &sl, &pa, &pb, &pkey, &pq,
+                        &paux, &accum, &aux_accum, &ctrl, &batch, &totals, &out_rgb, &out_aux})
+            b->release();
+    }
+};
+
+}  // namespace
+
+struct pt_scene {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    std::vector<uint32_t> host;  // the PTGP words
+    dbuf words, leaf, spill, work, io_rays, io_hits, counts;
+    ptd::SceneView view{};
+    int max_stack = 0;   // entries a lane's stack can reach in this scene
+    int tuned_stack = 0, tuned_blocks = 0;
+    render_bufs rb;
+    ~pt_scene() {
+        rb.release();
+        for (dbuf* b : {&words, &leaf, &spill, &work, &io_rays, &io_hits, &counts}) b->release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+struct section {
+    uint32_t offset, bytes;
+};
+
+float fbits(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Checks a PTGP v1 buffer (gpu_pack.hpp:177-203) well enough that no kernel
+// can read outside it: section bounds, every reachable tree link and leaf
+// range, primitive and material numbers. Returns the stack bound and builds
+// the leaf records.
+struct ptgp_check {
+    const uint32_t* w;
+    size_t n;
+    section sec[8];
+    uint32_t n_nodes, n_order, n_sph, n_tri, n_inst, n_mat, n_lights, n_large;
+    uint32_t flat_root, inst_root, n_small;
+    std::vector<float> leaf;   // 12 floats per order slot
+    int depth_flat = 0, depth_inst = 0, depth_blas = 0;
+
+    const uint32_t* at(int s) const { return w + sec[s].offset / 4; }
+    const float* node_f(uint32_t k) const { return reinterpret_cast<const float*>(at(0)) + 32 * k; }
+    const uint32_t* node_u(uint32_t k) const { return at(0) + 32 * k; }
+
+    static void bad(const std::string& m) { throw pt::status_error(PT_ERR_INVALID, "PTGP: " + m); }
+
+    void header() {
+        if (!w || n < 48) bad("buffer shorter than the 48-word header");
+        if (w[0] != 0x50475450u) bad("bad magic (expected \"PTGP\")");
+        if (w[1] != 1) bad("unsupported version");
+        const uint32_t elem[8] = {128, 4, 32, 112, 112, 48, 4, 4};
+        for (int s = 0; s < 8; ++s) {
+            sec[s] = {w[2 + 2 * s], w[3 + 2 * s]};
+            if (sec[s].offset % 16 || sec[s].offset < 192) bad("section offset not 16-byte aligned past the header");
+            if (uint64_t(sec[s].offset) + sec[s].bytes > 4 * uint64_t(n)) bad("section runs past the buffer");
+            if (sec[s].bytes % elem[s]) bad("section length not a whole number of records");
+        }
+        n_nodes = sec[0].bytes / 128;
+        n_order = sec[1].bytes / 4;
+        n_sph = sec[2].bytes / 32;
+        n_tri = sec[3].bytes / 112;
+        n_inst = sec[4].bytes / 112;
+        n_mat = sec[5].bytes / 48;
+        n_lights = sec[6].bytes / 4;
+        n_large = sec[7].bytes / 4;
+        flat_root = w[18];
+        inst_root = w[19];
+        n_small = w[20];
+        if (n_small > n_sph) bad("more small spheres than spheres");
+        if (n_nodes >= (1u << 28) || n_order >= (1u << 28)) bad("more than 2^28 nodes or leaf slots");
+        for (uint32_t m = 0; m < n_mat; ++m) {
+            const float type = fbits(at(5)[12 * m + 3]);
+            if (!(type == 0 || type == 1 || type == 2 || type == 3)) bad("material type not 0-3");
+        }
+        for (uint32_t s = 0; s < n_sph; ++s)
+            if (at(2)[8 * s + 4] >= n_mat) bad("sphere material out of range");
+        for (uint32_t t = 0; t < n_tri; ++t)
+            if (at(3)[28 * t + 3] >= n_mat) bad("triangle material out of range");
+        for (uint32_t k = 0; k < n_lights; ++k)
+            if (at(6)[k] >= n_small) bad("light is not a small sphere");
+        for (uint32_t k = 0; k < n_large; ++k)
+            if (at(7)[k] >= n_sph) bad("large sphere out of range");
+        for (uint32_t k = 0; k < n_inst; ++k) {
+            const uint32_t root = at(4)[28 * k + 24], mat = at(4)[28 * k + 25];
+            if (root != NONE && root >= n_nodes) bad("instance root out of range");
+            if (mat != NONE && mat >= n_mat) bad("instance material out of range");
+        }
+        leaf.assign(size_t(n_order) * 12, 0.0f);
+    }
+
+    void put_triangle(uint32_t slot, uint32_t t) {
+        const float* tr = reinterpret_cast<const float*>(at(3)) + 28 * t;
+        float* r = &leaf[12 * size_t(slot)];
+        r[0] = tr[0];
+        r[1] = tr[1];
+        r[2] = tr[2];
+        std::memcpy(&r[3], &t, 4);
+        r[4] = tr[4];
+        r[5] = tr[5];
+        r[6] = tr[6];
+        r[7] = 0;
+        r[8] = tr[8];
+        r[9] = tr[9];
+        r[10] = tr[10];
+        r[11] = 0;
+    }
+    void put_sphere(uint32_t slot, uint32_t s) {
+        const float* sp = reinterpret_cast<const float*>(at(2)) + 8 * s;
+        float* r = &leaf[12 * size_t(slot)];
+        r[0] = sp[0];
+        r[1] = sp[1];
+        r[2] = sp[2];
+        std::memcpy(&r[3], &s, 4);
+        r[4] = sp[3];
+        const uint32_t one = 1;
+        std::memcpy(&r[7], &one, 4);
+    }
+
+    // Walks one tree from root; kind 0 flat, 1 mesh, 2 instances. Returns its
+    // depth in 4-wide levels; collects mesh roots met in instance leaves.
+    int walk(uint32_t root, int kind, std::vector<uint8_t>& seen, std::vector<uint32_t>* blas_roots) {
+        if (root == NONE) return 0;
+        if (root >= n_nodes) bad("tree root out of range");
+        struct item {
+            uint32_t node;
+            int level;
+        };
+        std::vector<item> todo{{root, 1}};
+        std::vector<uint32_t> visited;
+        int depth = 0;
+        const float inf = std::numeric_limits<float>::infinity();
+        while (!todo.empty()) {
+            const item it = todo.back();
+            todo.pop_back();
+            if (seen[it.node]) bad("tree has a cycle or shared node");
+            seen[it.node] = 1;
+            visited.push_back(it.node);
+            depth = std::max(depth, it.level);
+            if (it.level > 64) bad("tree deeper than 64 levels");
+            const float* f = node_f(it.node);
+            const uint32_t* u = node_u(it.node);
+            for (int c = 0; c < 4; ++c) {
+                bool empty = true;
+                for (int a = 0; a < 6; ++a) empty = empty && f[4 * a + c] == inf;
+                const uint32_t first = u[24 + c], count = u[28 + c];
+                if (count > 0) {
+                    if (count > 15) bad("leaf holds more than 15 primitives");
+                    if (uint64_t(first) + count > n_order) bad("leaf range out of range");
+                    for (uint32_t i = first; i < first + count; ++i) {
+                        const uint32_t p = at(1)[i];
+                        if (kind == 0) {
+                            if (p < n_small) put_sphere(i, p);
+                            else if (p - n_small < n_tri) put_triangle(i, p - n_small);
+                            else bad("flat leaf primitive out of range");
+                        } else if (kind == 1) {
+                            if (p >= n_tri) bad("mesh leaf triangle out of range");
+                            put_triangle(i, p);
+                        } else {
+                            if (p >= n_inst) bad("instance leaf out of range");
+                            if (blas_roots) blas_roots->push_back(at(4)[28 * p + 24]);
+                        }
+                    }
+                } else if (!empty) {
+                    if (first >= n_nodes) bad("child node out of range");
+                    todo.push_back({first, it.level + 1});
+                }
+            }
+        }
+        for (uint32_t v : visited) seen[v] = 0;
+        return depth;
+    }
+
+    int run() {
+        header();
+        std::vector<uint8_t> seen(n_nodes, 0);
+        depth_flat = walk(flat_root, 0, seen, nullptr);
+        std::vector<uint32_t> roots;
+        depth_inst = walk(inst_root, 2, seen, &roots);
+        std::sort(roots.begin(), roots.end());
+        roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+        for (uint32_t r : roots) depth_blas = std::max(depth_blas, walk(r, 1, seen, nullptr));
+        const int flat_need = 3 * depth_flat + 1;
+        const int inst_need = depth_inst ? 3 * depth_inst + 1 + 3 * depth_blas + 1 : 0;
+        return std::max(flat_need, inst_need) + 4;
+    }
+};
+
+int choose_spb(int spp, size_t npix, int requested) {
+    if (requested > 0) {
+        if (spp % requested) throw pt::status_error(PT_ERR_INVALID, "samples_per_batch must divide spp");
+        return requested;
+    }
+    const size_t cap = std::max<size_t>(1, kTargetPaths / std::max<size_t>(1, npix));
+    int best = 1;
+    for (int d = 1; d <= spp; ++d)
+        if (spp % d == 0 && size_t(d) <= cap) best = d;
+    return best;
+}
+
+int stack_entries_for(pt_scene* s, int requested) {
+    int S = requested > 0 ? requested : (s->tuned_stack > 0 ? s->tuned_stack : kDefaultStack);
+    if (S != 8 && S != 16 && S != 32) throw pt::status_error(PT_ERR_INVALID, "stack_entries must be 8, 16 or 32");
+    return S;
+}
+
+int trace_grid_for(pt_scene* s, int S) {
+    const int per_cu = s->tuned_blocks > 0 ? s->tuned_blocks : ptd::trace_blocks_per_cu(S);
+    return std::max(1, s->cus * per_cu);
+}
+
+void ensure_spill(pt_scene* s, int grid) {
+    const size_t lanes = size_t(grid) * ptd::TRACE_BLOCK;
+    s->spill.ensure(lanes * size_t(s->max_stack) * sizeof(uint2));
+}
+
+}  // namespace
+
+extern "C" {
+
+pt_status pt_device_info(char* name, size_t name_len, int* compute_units) {
+    PT_GUARD({
+        int dev = 0;
+        HIPC(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIPC(hipGetDeviceProperties(&prop, dev));
+        if (name && name_len) {
+            std::strncpy(name, prop.gcnArchName, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (compute_units) *compute_units = prop.multiProcessorCount;
+    });
+}
+
+pt_status pt_ptgp_depth(const uint32_t* ptgp, size_t n_words, int* depth) {
+    if (!depth) return pt::fail(PT_ERR_INVALID, "pt_ptgp_depth: null output");
+    PT_GUARD({
+        ptgp_check chk{ptgp, n_words};
+        chk.run();
+        *depth = std::max({chk.depth_flat, chk.depth_inst, chk.depth_blas});
+    });
+}
+
+pt_status pt_scene_create(const uint32_t* ptgp, size_t n_words, pt_scene** out) {
+    if (!out) return pt::fail(PT_ERR_INVALID, "pt_scene_create: null output");
+    *out = nullptr;
+    pt_scene* s = nullptr;
+    try {
+        ptgp_check chk{ptgp, n_words};
+        const int need = chk.run();
+        if (need > 256) throw pt::status_error(PT_ERR_LIMIT, "PTGP: trees too deep for the traversal stack");
+        s = new pt_scene();
+        s->max_stack = need;
+        HIPC(hipGetDevice(&s->device));
+        hipDeviceProp_t prop;
+        HIPC(hipGetDeviceProperties(&prop, s->device));
+        s->cus = prop.multiProcessorCount;
+        HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->host.assign(ptgp, ptgp + n_words);
+        s->words.ensure(n_words * 4);
+        HIPC(hipMemcpy(s->words.p, ptgp, n_words * 4, hipMemcpyHostToDevice));
+        s->leaf.ensure(chk.leaf.size() * 4);
+        if (!chk.leaf.empty()) HIPC(hipMemcpy(s->leaf.p, chk.leaf.data(), chk.leaf.size() * 4, hipMemcpyHostToDevice));
+        s->work.ensure(64);
+        const uint32_t* dw = s->words.as<uint32_t>();
+        auto sec = [&](int k) { return dw + chk.sec[k].offset / 4; };
+        ptd::SceneView& v = s->view;
+        v.nodes = reinterpret_cast<const ptd::Node*>(sec(0));
+        v.order = sec(1);
+        v.leaf = s->leaf.as<float4>();
+        v.spheres = reinterpret_cast<const float4*>(sec(2));
+        v.tris = reinterpret_cast<const float4*>(sec(3));
+        v.inst = reinterpret_cast<const float4*>(sec(4));
+        v.mats = reinterpret_cast<const float4*>(sec(5));
+        v.lights = sec(6);
+        v.large = sec(7);
+        v.flat_root = chk.flat_root;
+        v.inst_root = chk.inst_root;
+        v.n_small = chk.n_small;
+        v.flags = ptgp[21];
+        v.n_lights = chk.n_lights;
+        v.n_large = chk.n_large;
+        for (int r = 0; r < 4; ++r)
+            v.cam[r] = make_float4(fbits(ptgp[22 + 4 * r]), fbits(ptgp[23 + 4 * r]), fbits(ptgp[24 + 4 * r]),
+                                   fbits(ptgp[25 + 4 * r]));
+        *out = s;
+        return PT_OK;
+    } catch (const pt::status_error& e) {
+        delete s;
+        return pt::fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        delete s;
+        return pt::fail(PT_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        delete s;
+        return pt::fail(PT_ERR_INVALID, e.what());
+    }
+}
+
+pt_status pt_scene_destroy(pt_scene* scene) {
+    if (!scene) return PT_OK;
+    PT_GUARD({
+        (void)hipStreamSynchronize(scene->stream);
+        delete scene;
+    });
+}
+
+pt_status pt_trace_device(pt_scene* s, const void* d_rays, size_t n, int any_hit, void* d_hits, float* elapsed_ms,
+                          uint64_t* visits) {
+    if (!s) return pt::fail(PT_ERR_INVALID, "pt_trace_device: null scene");
+    if (n == 0) {
+        if (elapsed_ms) *elapsed_ms = 0;
+        return PT_OK;
+    }
+    if (!d_rays || !d_hits) return pt::fail(PT_ERR_INVALID, "pt_trace_device: null buffer");
+    if (n >= (size_t(1) << 32) - 64) return pt::fail(PT_ERR_LIMIT, "pt_trace_device: more than 2^32 rays");
+    PT_GUARD({
+        const int S = stack_entries_for(s, 0);
+        const int grid = trace_grid_for(s, S);
+        ensure_spill(s, grid);
+        HIPC(hipMemsetAsync(s->work.p, 0, 4, s->stream));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (elapsed_ms) {
+            HIPC(hipEventCreate(&e0));
+            HIPC(hipEventCreate(&e1));
+            HIPC(hipEventRecord(e0, s->stream));
+        }
+        unsigned long long* counts = nullptr;
+        if (visits) {
+            s->counts.ensure(64);
+            HIPC(hipMemsetAsync(s->counts.p, 0, 48, s->stream));
+            counts = s->counts.as<unsigned long long>();
+        }
+        HIPC(ptd::launch_trace(s->view, static_cast<const float4*>(d_rays), static_cast<uint32_t>(n),
+                               static_cast<ptd::Hit*>(d_hits), any_hit != 0, s->spill.as<uint2>(), s->work.as<uint32_t>(),
+                               counts, S, grid, s->stream));
+        if (elapsed_ms) {
+            HIPC(hipEventRecord(e1, s->stream));
+            HIPC(hipEventSynchronize(e1));
+            HIPC(hipEventElapsedTime(elapsed_ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        if (visits) {
+            unsigned long long v[6];
+            HIPC(hipMemcpyAsync(v, s->counts.p, 48, hipMemcpyDeviceToHost, s->stream));
+            HIPC(hipStreamSynchronize(s->stream));
+            for (int k = 0; k < 6; ++k) visits[k] = v[k];
+        }
+        HIPC(hipStreamSynchronize(s->stream));
+    });
+}
+
+pt_status pt_trace(pt_scene* s, const pt_ray* rays, size_t n, int any_hit, pt_hit* hits) {
+    if (!s) return pt::fail(PT_ERR_INVALID, "pt_trace: null scene");
+    if (n == 0) return PT_OK;
+    if (!rays || !hits) return pt::fail(PT_ERR_INVALID, "pt_trace: null buffer");
+    static_assert(sizeof(pt_ray) == 32 && sizeof(pt_hit) == 24 && sizeof(ptd::Hit) == 24, "ABI record sizes");
+    PT_GUARD({
+        s->io_rays.ensure(n * sizeof(pt_ray));
+        s->io_hits.ensure(n * sizeof(pt_hit));
+        HIPC(hipMemcpyAsync(s->io_rays.p, rays, n * sizeof(pt_ray), hipMemcpyHostToDevice, s->stream));
+        const pt_status st = pt_trace_device(s, s->io_rays.p, n, any_hit, s->io_hits.p, nullptr, nullptr);
+        if (st != PT_OK) throw pt::status_error(st, pt_last_error());
+        HIPC(hipMemcpyAsync(hits, s->io_hits.p, n * sizeof(pt_hit), hipMemcpyDeviceToHost, s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+    });
+}
+
+pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float* aux, pt_render_stats* stats) {
+    if (!s || !prm || !rgb) return pt::fail(PT_ERR_INVALID, "pt_render: null argument");
+    PT_GUARD({
+        const pt_render_params& P = *prm;
+        if (P.width < 2 || P.height < 2 || P.width > 32768 || P.height > 32768)
+            throw pt::status_error(PT_ERR_INVALID, "pt_render: width and height must be in [2, 32768]");
+        if (P.spp < 1) throw pt::status_error(PT_ERR_INVALID, "pt_render: spp must be at least 1");
+        if (P.max_bounces < 1 || P.max_bounces >= ptd::MAX_BOUNCES)
+            throw pt::status_error(PT_ERR_INVALID, "pt_render: max_bounces must be in [1, 63]");
+        int j_lo = P.row_begin, j_hi = P.row_end;
+        if (j_lo == 0 && j_hi == 0) j_hi = P.height;
+        if (j_lo < 0 || j_hi > P.height || j_lo >= j_hi)
+            throw pt::status_error(PT_ERR_INVALID, "pt_render: row range outside the image");
+        if (P.first_sample < 0) throw pt::status_error(PT_ERR_INVALID, "pt_render: first_sample must be >= 0");
+        const size_t npix = size_t(P.width) * size_t(j_hi - j_lo);
+        const int spb = choose_spb(P.spp, npix, P.samples_per_batch);
+        const size_t paths = npix * size_t(spb);
+        if (paths >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render: batch too large");
+        const int S = stack_entries_for(s, P.stack_entries);
+        const int trace_grid = trace_grid_for(s, S);
+        const int shade_grid = std::max(1, s->cus * 8);
+        ensure_spill(s, trace_grid);
+        const bool want_aux = aux != nullptr;
+
+        render_bufs& B = s->rb;
+        for (int k = 0; k < 2; ++k) {
+            B.ro[k].ensure(paths * 16);
+            B.rd[k].ensure(paths * 16);
+            B.rpath[k].ensure(paths * 4);
+        }
+        B.hits.ensure(paths * sizeof(ptd::Hit));
+        B.so.ensure(paths * 16);
+        B.sd.ensure(paths * 16);
+        B.sl.ensure(paths * 16);
+        B.pa.ensure(paths * 16);
+        B.pb.ensure(paths * 16);
+        B.pkey.ensure(paths * 8);
+        B.pq.ensure(paths * 8);
+        if (want_aux) {
+            B.paux.ensure(paths * 32);
+            B.aux_accum.ensure(npix * 32);
+            B.out_aux.ensure(npix * 7 * 4);
+        }
+        B.accum.ensure(npix * 16);
+        B.ctrl.ensure(sizeof(ptd::Ctrl) * (ptd::MAX_BOUNCES + 1));
+        B.batch.ensure(16);
+        B.totals.ensure(32);
+        B.out_rgb.ensure(npix * 12);
+        HIPC(hipMemsetAsync(B.ctrl.p, 0, sizeof(ptd::Ctrl) * (ptd::MAX_BOUNCES + 1), s->stream));
+
+        ptd::FrameParams fp{};
+        fp.sc = s->view;
+        if (P.flags & PT_RENDER_NO_NEE) fp.sc.flags &= ~2u;
+        fp.W = P.width;
+        fp.H = P.height;
+        fp.spp = P.spp;
+        fp.depth = P.max_bounces;
+        fp.j_lo = j_lo;
+        fp.j_hi = j_hi;
+        fp.seed = P.seed;
+        fp.first_sample = P.first_sample;
+        fp.spb = spb;
+        fp.P = static_cast<uint32_t>(paths);
+        fp.npix = static_cast<uint32_t>(npix);
+        fp.nee = (fp.sc.flags & 2u) ? 1 : 0;
+        fp.want_aux = want_aux ? 1 : 0;
+        for (int k = 0; k < 2; ++k) {
+            fp.ro[k] = B.ro[k].as<float4>();
+            fp.rd[k] = B.rd[k].as<float4>();
+            fp.rpath[k] = B.rpath[k].as<uint32_t>();
+        }
+        fp.hits = B.hits.as<ptd::Hit>();
+        fp.so = B.so.as<float4>();
+        fp.sd = B.sd.as<float4>();
+        fp.sl = B.sl.as<float4>();
+        fp.pa = B.pa.as<float4>();
+        fp.pb = B.pb.as<float4>();
+        fp.pkey = B.pkey.as<uint2>();
+        fp.pq = B.pq.as<uint2>();
+        fp.paux = want_aux ? B.paux.as<float4>() : nullptr;
+        fp.accum = B.accum.as<float4>();
+        fp.aux_accum = want_aux ? B.aux_accum.as<float4>() : nullptr;
+        fp.ctrl = B.ctrl.as<ptd::Ctrl>();
+        fp.batch = B.batch.as<uint32_t>();
+        fp.totals = B.totals.as<unsigned long long>();
+        fp.spill = s->spill.as<uint2>();
+        const bool counting = stats && (P.flags & PT_RENDER_COUNT);
+        if (counting) {
+            s->counts.ensure(64);
+            HIPC(hipMemsetAsync(s->counts.p, 0, 48, s->stream));
+            fp.visits = s->counts.as<unsigned long long>();
+        }
+
+        const int batches = P.spp / spb;
+        hipEvent_t e0, e1;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        HIPC(ptd::launch_clear_frame(fp, s->stream));
+        HIPC(hipEventRecord(e0, s->stream));
+        double extend_ms = 0, shadow_ms = 0;
+        const bool timing = stats && (P.flags & PT_RENDER_TIMING);
+        if (timing) {
+            // Eager, with an event pair around every traversal launch.
+            std::vector<hipEvent_t> ev(4 * P.max_bounces);
+            for (auto& e : ev) HIPC(hipEventCreate(&e));
+            for (int b = 0; b < batches; ++b) {
+                HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, ev.data(), ev.data() + 2 * P.max_bounces));
+                HIPC(hipStreamSynchronize(s->stream));
+                for (int k = 0; k < P.max_bounces; ++k) {
+                    float ms = 0;
+                    HIPC(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+                    extend_ms += ms;
+                    HIPC(hipEventElapsedTime(&ms, ev[2 * P.max_bounces + 2 * k], ev[2 * P.max_bounces + 2 * k + 1]));
+                    shadow_ms += ms;
+                }
+            }
+            for (auto& e : ev) (void)hipEventDestroy(e);
+        } else if ((P.flags & PT_RENDER_NO_GRAPH) || counting) {
+            for (int b = 0; b < batches; ++b)
+                HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr, nullptr));
+        } else {
+            // The batch's whole bounce loop as one graph, replayed per batch;
+            // kernels read queue sizes and the batch number from HBM.
+            hipGraph_t graph;
+            hipGraphExec_t exec;
+            HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr, nullptr);
+            const hipError_t ce = hipStreamEndCapture(s->stream, &graph);
+            HIPC(le);
+            HIPC(ce);
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPC(ie);
+            for (int b = 0; b < batches; ++b) {
+                const hipError_t ge = hipGraphLaunch(exec, s->stream);
+                if (ge != hipSuccess) {
+                    (void)hipGraphExecDestroy(exec);
+                    HIPC(ge);
+                }
+            }
+            HIPC(hipStreamSynchronize(s->stream));
+            (void)hipGraphExecDestroy(exec);
+        }
+        HIPC(hipEventRecord(e1, s->stream));
+        HIPC(hipEventSynchronize(e1));
+        float render_ms = 0;
+        HIPC(hipEventElapsedTime(&render_ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+
+        HIPC(ptd::launch_resolve(fp, B.out_rgb.as<float>(), want_aux ? B.out_aux.as<float>() : nullptr, s->stream));
+        HIPC(hipMemcpyAsync(rgb, B.out_rgb.p, npix * 12, hipMemcpyDeviceToHost, s->stream));
+        if (want_aux) HIPC(hipMemcpyAsync(aux, B.out_aux.p, npix * 28, hipMemcpyDeviceToHost, s->stream));
+        unsigned long long totals[3] = {0, 0, 0};
+        HIPC(hipMemcpyAsync(totals, B.totals.p, sizeof totals, hipMemcpyDeviceToHost, s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+        if (stats) {
+            stats->rays = totals[0] + totals[1];
+            stats->primary_rays = totals[2];
+            stats->shadow_rays = totals[1];
+            stats->render_ms = render_ms;
+            stats->extend_ms = extend_ms;
+            stats->shadow_ms = shadow_ms;
+            unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+            if (counting) HIPC(hipMemcpy(v, s->counts.p, 48, hipMemcpyDeviceToHost));
+            for (int k = 0; k < 6; ++k) stats->visits[k] = v[k];
+        }
+    });
+}
+
+}  // extern "C"

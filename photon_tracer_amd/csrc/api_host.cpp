@@ -1,0 +1,69 @@
+// C ABI: error reporting and the host-only entry points (scene generation
+// and packing). Device entry points are in api.hip.
+#include <cstdlib>
+#include <new>
+#include <string>
+
+#include "../../include/photon_tracer.h"
+#include "host.hpp"
+#include "status.hpp"
+
+namespace pt {
+
+thread_local std::string g_last_error;
+
+pt_status fail(pt_status s, const std::string& msg) {
+    g_last_error = msg;
+    return s;
+}
+
+uint32_t* copy_out(const std::vector<uint32_t>& v, size_t* n) {
+    uint32_t* p = static_cast<uint32_t*>(std::malloc(v.size() * 4 + 4));
+    if (!p) throw std::bad_alloc();
+    if (!v.empty()) std::memcpy(p, v.data(), v.size() * 4);
+    *n = v.size();
+    return p;
+}
+
+}  // namespace pt
+
+extern "C" {
+
+const char* pt_last_error(void) { return pt::g_last_error.c_str(); }
+
+const char* pt_version(void) { return "photon_tracer_amd 0.1 (gfx950)"; }
+
+void pt_free(void* p) { std::free(p); }
+
+pt_status pt_generate(int config, uint64_t seed, double scale, uint32_t** ptsd, size_t* n_words) {
+    if (!ptsd || !n_words) return pt::fail(PT_ERR_INVALID, "pt_generate: null output pointer");
+    PT_GUARD({
+        const pt::scene_desc s = pt::generate_config(config, seed, scale);
+        *ptsd = pt::copy_out(pt::ptsd_write(s), n_words);
+    });
+}
+
+pt_status pt_generate_job(uint64_t seed, double scale, pt_job_desc* desc, uint32_t** ptsd, size_t* n_words) {
+    if (!desc || !ptsd || !n_words) return pt::fail(PT_ERR_INVALID, "pt_generate_job: null output pointer");
+    PT_GUARD({
+        pt::job_desc d{};
+        const pt::scene_desc s = pt::generate_job(seed, scale, d);
+        desc->kind = d.kind;
+        desc->width = d.width;
+        desc->height = d.height;
+        desc->spp = d.spp;
+        desc->primitives = d.primitives;
+        desc->seed = d.seed;
+        *ptsd = pt::copy_out(pt::ptsd_write(s), n_words);
+    });
+}
+
+pt_status pt_pack(const uint32_t* ptsd, size_t n_words, uint32_t** ptgp, size_t* ptgp_words) {
+    if (!ptsd || !ptgp || !ptgp_words) return pt::fail(PT_ERR_INVALID, "pt_pack: null pointer");
+    PT_GUARD({
+        const pt::scene_desc s = pt::ptsd_read(ptsd, n_words);
+        *ptgp = pt::copy_out(pt::pack_ptgp(s), ptgp_words);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Launch interface between the C ABI (api.hip) and the kernel translation
+// units (trace.hip, render.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace ptd {
+
+// Per-bounce control block in HBM: queue sizes and the work counters the
+// persistent kernels take items from. Zeroed by k_finish at the end of every
+// batch, so a captured graph can be replayed.
+struct Ctrl {
+    uint32_t n_rays;      // closest-hit queue size of this bounce
+    uint32_t n_shadow;    // shadow queue size of this bounce
+    uint32_t n_backface;  // light samples that count as rays but need no trace
+    uint32_t w_extend;    // next item of the closest-hit queue to hand out
+    uint32_t w_shadow;    // next item of the shadow queue to hand out
+    uint32_t pad[3];
+};
+constexpr int MAX_BOUNCES = 64;
+
+struct FrameParams {
+    SceneView sc;
+    int W, H, spp, depth;
+    int j_lo, j_hi;           // rows rendered: j in [j_lo, j_hi), counted from the bottom
+    uint64_t seed;
+    int first_sample, spb;    // samples per pixel per batch
+    uint32_t P, npix;         // paths per batch, pixels in the band
+    int nee, want_aux;
+    // ray queues (double buffered by bounce parity): (o, tmin), (d, tmax), path
+    float4* ro[2];
+    float4* rd[2];
+    uint32_t* rpath[2];
+    Hit* hits;
+    // shadow queue: (o, tmin), (d, tmax), (L, path)
+    float4* so;
+    float4* sd;
+    float4* sl;
+    // path state: (throughput, rng counter), (radiance, flags), rng key, (pixel seed, sample index)
+    float4* pa;
+    float4* pb;
+    uint2* pkey;
+    uint2* pq;
+    float4* paux;             // 2 per path: (first albedo, 0), (first normal, 0)
+    float4* accum;            // per pixel: rgb sum
+    float4* aux_accum;        // 2 per pixel: (albedo sum, lum^2 sum), (normal sum, 0)
+    Ctrl* ctrl;               // MAX_BOUNCES + 1 entries
+    uint32_t* batch;          // batch number, advanced by k_finish
+    unsigned long long* totals;  // [0] closest-hit rays, [1] shadow rays, [2] primary rays
+    uint2* spill;             // traversal stack spill area
+    unsigned long long* visits;  // null, or six totals the closest-hit stage counts into
+};
+
+// Traversal kernels for ray batches (pt_trace).
+// counts: null, or six 64-bit visit totals (Visits) to add the batch's into.
+hipError_t launch_trace(const SceneView& sc, const float4* rays, uint32_t n, Hit* hits, bool any_hit, uint2* spill,
+                        uint32_t* work, unsigned long long* counts, int stack_entries, int grid, hipStream_t stream);
+// Occupancy of the traversal kernels: blocks of 256 per CU.
+int trace_blocks_per_cu(int stack_entries);
+constexpr int TRACE_BLOCK = 256;
+
+// The bounce loop of one batch of paths.
+hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
+                        hipEvent_t* ev_extend, hipEvent_t* ev_shadow);
+hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStream_t stream);
+hipError_t launch_clear_frame(const FrameParams& fp, hipStream_t stream);
+
+}  // namespace ptd

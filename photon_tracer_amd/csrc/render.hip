@@ -1,0 +1,486 @@
+// The wavefront path loop: render_row_linear / ray_color (renderer.hpp:100-149,
+// 366-416) split into stages over SoA queues in HBM.
+//
+//   raygen   camera rays for one batch of paths (pixels x samples)
+//   extend   closest hit of every queued ray            (geometry::hit)
+//   shade    emission, scatter, light sample, roulette  (ray_color's body)
+//   shadow   any hit of every queued light sample       (geometry::occluded)
+//   ...      extend/shade/shadow again for each bounce, up to max_bounces
+//   accumulate  per pixel, the batch's samples in sample order
+//   finish   ray counts into the totals, control blocks zeroed
+//
+// A path's state is indexed by its path number, not its queue slot, and every
+// random number it draws comes from its own counter-based stream, so the
+// image does not depend on the order in which queues fill. Queues are filled
+// by wave-aggregated appends (ballot, popcount, one atomic per wave).
+#include "kernels.hpp"
+#include "traverse.hpp"
+
+namespace ptd {
+
+namespace {
+
+constexpr int SHADE_BLOCK = 256;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
+
+// Slot for this lane in a queue whose length is *counter, for the lanes of the
+// wave where pred holds. Called by every lane of the wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t cnt = static_cast<uint32_t>(__popcll(mask));
+    uint32_t base = 0;
+    if (lane_id() == 0 && cnt) base = atomicAdd(counter, cnt);
+    base = __shfl(base, 0, 64);
+    const unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+    return base + static_cast<uint32_t>(__popcll(below));
+}
+
+// sample_2d (sampler.hpp:107-124): QMC for pairs up to 2 * qmc_bounces,
+// else two numbers from the path's stream.
+__device__ __forceinline__ void sample_2d(uint32_t pseed, uint32_t index, int pair, uint64_t key, uint32_t& cnt,
+                                          float& a, float& b) {
+    if (pair > 2 * QMC_BOUNCES) {
+        a = rand01(key, cnt);
+        b = rand01(key, cnt);
+        return;
+    }
+    qmc_pair(pseed, index, pair, a, b);
+}
+
+__device__ __forceinline__ f3 reflect(f3 v, f3 n) { return v - n * (2 * dot(v, n)); }
+
+__device__ __forceinline__ f3 refract(f3 uv, f3 n, float ratio) {
+    const float cos_theta = fminf(dot(-uv, n), 1.0f);
+    const f3 r_perp = (uv + n * cos_theta) * ratio;
+    const f3 r_parallel = n * -sqrtf(fabsf(1 - dot(r_perp, r_perp)));
+    return r_perp + r_parallel;
+}
+
+__device__ __forceinline__ float schlick(float cosine, float ratio) {
+    float r0 = (1 - ratio) / (1 + ratio);
+    r0 = r0 * r0;
+    const float x = 1 - cosine, x2 = x * x;
+    return r0 + (1 - r0) * x2 * x2 * x;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_raygen(FrameParams fp) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= fp.P) return;
+    const uint32_t batch = *fp.batch;
+    const uint32_t pixel = p / fp.spb, s = p % fp.spb;
+    const int i = static_cast<int>(pixel % fp.W);
+    const int k = static_cast<int>(pixel / fp.W);   // row within the band, top first
+    const int j = fp.j_hi - 1 - k;
+    const uint32_t index = static_cast<uint32_t>(fp.first_sample) + batch * fp.spb + s;
+    const uint32_t pseed = pixel_seed(fp.seed, i, j);
+    float du, dv;
+    qmc_pair(pseed, index, 0, du, dv);
+    // renderer.hpp:383-385, camera.hpp:45-47
+    const float u = (static_cast<float>(i) + du) / static_cast<float>(fp.W - 1);
+    const float v = (static_cast<float>(j) + dv) / static_cast<float>(fp.H - 1);
+    const f3 origin = xyz(fp.sc.cam[0]);
+    const f3 d = xyz(fp.sc.cam[1]) + xyz(fp.sc.cam[2]) * u + xyz(fp.sc.cam[3]) * v - origin;
+    fp.ro[0][p] = make_float4(origin.x, origin.y, origin.z, T_MIN);
+    fp.rd[0][p] = make_float4(d.x, d.y, d.z, T_MAX);
+    fp.rpath[0][p] = p;
+    fp.pa[p] = make_float4(1, 1, 1, __uint_as_float(0u));
+    fp.pb[p] = make_float4(0, 0, 0, __uint_as_float(0u));
+    const uint64_t key = path_key(fp.seed, i, j, index);
+    fp.pkey[p] = make_uint2(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32));
+    fp.pq[p] = make_uint2(pseed, index);
+    if (p == 0) fp.ctrl[0].n_rays = fp.P;
+}
+
+template <int S, bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(FrameParams fp, int b) {
+    __shared__ uint2 lds[S * TRACE_BLOCK];
+    auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
+    Visits vis;
+    Ctrl* c = fp.ctrl + b;
+    const uint32_t n = c->n_rays;
+    const float4* ro = fp.ro[b & 1];
+    const float4* rd = fp.rd[b & 1];
+    while (true) {
+        const uint32_t base = wave_fetch(&c->w_extend, 64);
+        if (base >= n) break;
+        const uint32_t q = base + lane_id();
+        if (q < n) {
+            const float4 a = ro[q], bb = rd[q];
+            Hit h;
+            trace_ray<false, COUNT>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st, &vis);
+            fp.hits[q] = h;
+        }
+    }
+    if (COUNT) add_visits(fp.visits, vis);
+}
+
+template <int S>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(FrameParams fp, int b) {
+    __shared__ uint2 lds[S * TRACE_BLOCK];
+    auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
+    Ctrl* c = fp.ctrl + b;
+    const uint32_t n = c->n_shadow;
+    while (true) {
+        const uint32_t base = wave_fetch(&c->w_shadow, 64);
+        if (base >= n) break;
+        const uint32_t q = base + lane_id();
+        if (q < n) {
+            const float4 a = fp.so[q], bb = fp.sd[q];
+            Hit h;
+            if (!trace_ray<true>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
+                const float4 l = fp.sl[q];
+                const uint32_t p = __float_as_uint(l.w);
+                float4 r = fp.pb[p];
+                r.x = r.x + l.x;
+                r.y = r.y + l.y;
+                r.z = r.z + l.z;
+                fp.pb[p] = r;
+            }
+        }
+    }
+}
+
+// ray_color's loop body for bounce b (renderer.hpp:109-146).
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
+    const SceneView& sc = fp.sc;
+    Ctrl* c = fp.ctrl + b;
+    const uint32_t n = c->n_rays;
+    const int cur = b & 1, nxt = cur ^ 1;
+    const uint32_t stride = gridDim.x * SHADE_BLOCK;
+    for (uint32_t wbase = blockIdx.x * SHADE_BLOCK + (threadIdx.x & ~63u); wbase < n; wbase += stride) {
+        const uint32_t q = wbase + lane_id();
+        const bool active = q < n;
+        bool cont = false, shadow = false;
+        f3 next_o = mk(0, 0, 0), next_d = mk(0, 0, 0), sh_d = mk(0, 0, 0), sh_l = mk(0, 0, 0);
+        float sh_tmax = 0;
+        bool backface = false;
+        uint32_t p = 0;
+        if (active) {
+            p = fp.rpath[cur][q];
+            const float4 o4 = fp.ro[cur][q], d4 = fp.rd[cur][q];
+            const Hit h = fp.hits[q];
+            const float4 A = fp.pa[p], B = fp.pb[p];
+            const uint2 kk = fp.pkey[p], pq = fp.pq[p];
+            const uint64_t key = (static_cast<uint64_t>(kk.y) << 32) | kk.x;
+            f3 thr = xyz(A), rad = xyz(B);
+            uint32_t cnt = __float_as_uint(A.w);
+            const bool sampled = (__float_as_uint(B.w) & 1u) != 0;
+            uint32_t flags = 0;
+            const f3 o = xyz(o4), d = xyz(d4);
+            if (h.kind == 0) {
+                // renderer.hpp:24-29
+                f3 bg = mk(0, 0, 0);
+                if (sc.flags & 1u) {
+                    const f3 unit = normalize(d);
+                    const float t = 0.5f * (unit.y + 1);
+                    bg = mk(1, 1, 1) * (1 - t) + mk(0.5f, 0.7f, 1.0f) * t;
+                }
+                rad = rad + mul(thr, bg);
+                if (fp.want_aux && b == 0) {
+                    fp.paux[2 * p] = make_float4(1, 1, 1, 0);
+                    fp.paux[2 * p + 1] = make_float4(0, 0, 0, 0);
+                }
+            } else {
+                // The hit record (geometry.hpp:135-151, sphere.hpp:50-58,
+                // triangle.hpp:63-74, geometry.hpp:231-244).
+                const float t = h.t;
+                const f3 pnt = o + d * t;
+                f3 nrm;
+                bool front;
+                uint32_t mat;
+                bool sampled_light = false;
+                if (h.kind <= 2) {
+                    const float4 cr = sc.spheres[2 * h.prim];
+                    mat = __float_as_uint(sc.spheres[2 * h.prim + 1].x);
+                    const f3 outward = (pnt - xyz(cr)) * (1 / cr.w);
+                    front = dot(d, outward) < 0;
+                    nrm = front ? outward : -outward;
+                    if (h.kind == 2 && fp.nee) {
+                        const float4 em = sc.mats[3 * mat + 1];
+                        sampled_light = em.x > 0 || em.y > 0 || em.z > 0;
+                    }
+                } else {
+                    const float4* tr = sc.tris + 7 * h.prim;
+                    mat = __float_as_uint(tr[0].w);
+                    const f3 n0 = xyz(tr[3]), n1 = xyz(tr[4]), n2 = xyz(tr[5]), fn = xyz(tr[6]);
+                    const float b0 = 1 - h.u - h.v;
+                    const f3 sn = normalize(n0 * b0 + n1 * h.u + n2 * h.v);
+                    if (h.inst == NONE) {
+                        front = dot(d, fn) < 0;
+                        nrm = front ? sn : -sn;
+                    } else {
+                        const float4* in = sc.inst + 7 * h.inst;
+                        const float4 r0 = in[3], r1 = in[4], r2 = in[5];
+                        const f3 ld = mk(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+                                         r2.x * d.x + r2.y * d.y + r2.z * d.z);
+                        front = dot(ld, fn) < 0;
+                        const f3 ln = front ? sn : -sn;
+                        // the transpose of to_object (transform.hpp:23-27)
+                        nrm = normalize(mk(r0.x * ln.x + r1.x * ln.y + r2.x * ln.z, r0.y * ln.x + r1.y * ln.y + r2.y * ln.z,
+                                           r0.z * ln.x + r1.z * ln.y + r2.z * ln.z));
+                        const uint32_t im = __float_as_uint(in[6].y);
+                        if (im != NONE) mat = im;
+                    }
+                }
+                const float4 m0 = sc.mats[3 * mat], m1 = sc.mats[3 * mat + 1], m2 = sc.mats[3 * mat + 2];
+                const f3 emission = xyz(m1);
+                if (front && !(sampled && sampled_light)) rad = rad + mul(thr, emission);
+
+                // material::scatter (material.hpp:59-137)
+                const int type = static_cast<int>(m0.w);
+                f3 att = xyz(m0), dir = mk(0, 0, 0);
+                bool ok = true;
+                if (type == 0) {
+                    float a, bq;
+                    sample_2d(pq.x, pq.y, 1 + 2 * b, key, cnt, a, bq);
+                    const float r = sqrtf(a), phi = 2 * PI_F * bq;
+                    const f3 nn = nrm;
+                    const float sign = copysignf(1.0f, nn.z);
+                    const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
+                    const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
+                    const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
+                    dir = t1 * (r * cosf(phi)) + t2 * (r * sinf(phi)) + nn * sqrtf(fmaxf(0.0f, 1 - a));
+                } else if (type == 1) {
+                    const f3 reflected = reflect(normalize(d), nrm);
+                    f3 ball = mk(0, 0, 0);
+                    for (int tries = 0; tries < 64; ++tries) {  // vec3.hpp:121-127
+                        const float x = -1 + 2 * rand01(key, cnt);
+                        const float y = -1 + 2 * rand01(key, cnt);
+                        const float z = -1 + 2 * rand01(key, cnt);
+                        ball = mk(x, y, z);
+                        if (dot(ball, ball) < 1) break;
+                    }
+                    dir = reflected + ball * m1.w;
+                    ok = dot(dir, nrm) > 0;
+                } else if (type == 2) {
+                    att = mk(1, 1, 1);
+                    const float ior = m2.x;
+                    const float ratio = front ? (1 / ior) : ior;
+                    const f3 ud = normalize(d);
+                    const float cos_theta = fminf(dot(-ud, nrm), 1.0f);
+                    const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
+                    const bool cannot_refract = ratio * sin_theta > 1;
+                    const bool refl = cannot_refract || schlick(cos_theta, ratio) > rand01(key, cnt);
+                    dir = refl ? reflect(ud, nrm) : refract(ud, nrm, ratio);
+                } else {
+                    ok = false;
+                }
+                if (fp.want_aux && b == 0) {
+                    const f3 fa = ok && type != 2 ? att : mk(1, 1, 1);
+                    fp.paux[2 * p] = make_float4(fa.x, fa.y, fa.z, 0);
+                    fp.paux[2 * p + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0);
+                }
+                if (ok) {
+                    const bool nee = type == 0 && fp.nee && sc.n_lights > 0 && b + 1 < fp.depth;
+                    if (nee) {
+                        // direct_light (renderer.hpp:41-80)
+                        const uint32_t nl = sc.n_lights;
+                        const uint32_t pick = min(static_cast<uint32_t>(rand01(key, cnt) * nl), nl - 1);
+                        const uint32_t ls = sc.lights[pick];
+                        const float4 lcr = sc.spheres[2 * ls];
+                        const uint32_t lmat = __float_as_uint(sc.spheres[2 * ls + 1].x);
+                        const f3 to_center = xyz(lcr) - pnt;
+                        const float dist2 = dot(to_center, to_center), r2 = lcr.w * lcr.w;
+                        float s1, s2;
+                        if (dist2 > r2) {
+                            const float sin2_max = r2 / dist2;
+                            const float cos_max = sqrtf(1 - sin2_max);
+                            const float omc = sin2_max / (1 + cos_max);
+                            const f3 w = to_center * (1 / sqrtf(dist2));
+                            const f3 ax = fabsf(w.x) > 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
+                            const f3 uu = normalize(cross(w, ax)), vv = cross(w, uu);
+                            sample_2d(pq.x, pq.y, 2 + 2 * b, key, cnt, s1, s2);
+                            const float cos_t = 1 - s1 * omc;
+                            const float sin_t = sqrtf(fmaxf(0.0f, 1 - cos_t * cos_t));
+                            const float phi = 2 * PI_F * s2;
+                            const f3 ldir = uu * (cosf(phi) * sin_t) + vv * (sinf(phi) * sin_t) + w * cos_t;
+                            const float cos_surface = dot(ldir, nrm);
+                            float t_light = __builtin_huge_valf();
+                            if (cos_surface > 0 && sphere_hit_f(lcr, pnt, ldir, 0.001f, t_light)) {
+                                const f3 lp = pnt + ldir * t_light;
+                                const f3 outward = (lp - xyz(lcr)) * (1 / lcr.w);
+                                if (dot(ldir, outward) < 0) {
+                                    const float pdf = 1 / (2 * PI_F * omc);
+                                    const f3 contrib = xyz(sc.mats[3 * lmat + 1]) *
+                                                       (cos_surface / PI_F / pdf * static_cast<float>(nl));
+                                    shadow = true;
+                                    sh_d = ldir;
+                                    sh_tmax = t_light * 0.9999f;
+                                    sh_l = mul(mul(thr, att), contrib);
+                                } else {
+                                    backface = true;
+                                }
+                            }
+                        }
+                    }
+                    flags = nee ? 1u : 0u;
+                    thr = mul(thr, att);
+                    bool alive = true;
+                    if (b + 1 >= 3) {  // roulette_after (renderer.hpp:22,140-146)
+                        const float pm = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                        if (pm < 1) {
+                            if (rand01(key, cnt) >= pm) alive = false;
+                            else thr = thr * (1 / pm);
+                        }
+                    }
+                    if (alive && b + 1 < fp.depth) {
+                        cont = true;
+                        next_o = pnt;
+                        next_d = dir;
+                    }
+                }
+                next_o = pnt;
+            }
+            fp.pa[p] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(cnt));
+            fp.pb[p] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(flags));
+        }
+        // Every lane of the wave reaches the appends.
+        const uint32_t slot = wave_append(&fp.ctrl[b + 1].n_rays, cont);
+        if (cont) {
+            fp.ro[nxt][slot] = make_float4(next_o.x, next_o.y, next_o.z, T_MIN);
+            fp.rd[nxt][slot] = make_float4(next_d.x, next_d.y, next_d.z, T_MAX);
+            fp.rpath[nxt][slot] = p;
+        }
+        const uint32_t sslot = wave_append(&c->n_shadow, shadow);
+        if (shadow) {
+            fp.so[sslot] = make_float4(next_o.x, next_o.y, next_o.z, 0.001f);
+            fp.sd[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_tmax);
+            fp.sl[sslot] = make_float4(sh_l.x, sh_l.y, sh_l.z, __uint_as_float(p));
+        }
+        const unsigned long long bf = __ballot(backface);
+        if (lane_id() == 0 && bf) atomicAdd(&c->n_backface, static_cast<uint32_t>(__popcll(bf)));
+    }
+}
+
+// Per pixel, the batch's samples added in sample order onto the running sum
+// (renderer.hpp:379-396).
+__global__ __launch_bounds__(256) void k_accumulate(FrameParams fp) {
+    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= fp.npix) return;
+    float4 acc = fp.accum[px];
+    f3 sum = xyz(acc);
+    float4 a0 = make_float4(0, 0, 0, 0), a1 = make_float4(0, 0, 0, 0);
+    if (fp.want_aux) {
+        a0 = fp.aux_accum[2 * px];
+        a1 = fp.aux_accum[2 * px + 1];
+    }
+    for (int s = 0; s < fp.spb; ++s) {
+        const uint32_t p = px * fp.spb + s;
+        const f3 c = xyz(fp.pb[p]);
+        sum = sum + c;
+        if (fp.want_aux) {
+            const float4 fa = fp.paux[2 * p], fn = fp.paux[2 * p + 1];
+            a0.x = a0.x + fa.x;
+            a0.y = a0.y + fa.y;
+            a0.z = a0.z + fa.z;
+            a1.x = a1.x + fn.x;
+            a1.y = a1.y + fn.y;
+            a1.z = a1.z + fn.z;
+            const float lum = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;  // renderer.hpp:364
+            a0.w += lum * lum;
+        }
+    }
+    fp.accum[px] = make_float4(sum.x, sum.y, sum.z, 0);
+    if (fp.want_aux) {
+        fp.aux_accum[2 * px] = a0;
+        fp.aux_accum[2 * px + 1] = a1;
+    }
+}
+
+__global__ void k_finish(FrameParams fp) {
+    if (threadIdx.x != 0) return;
+    unsigned long long rays = 0, shadow = 0;
+    for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) {
+        rays += fp.ctrl[b].n_rays;
+        shadow += fp.ctrl[b].n_shadow + fp.ctrl[b].n_backface;
+    }
+    fp.totals[0] += rays;
+    fp.totals[1] += shadow;
+    fp.totals[2] += fp.ctrl[0].n_rays;
+    for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
+    *fp.batch += 1;
+}
+
+__global__ __launch_bounds__(256) void k_clear_frame(FrameParams fp) {
+    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px < fp.npix) {
+        fp.accum[px] = make_float4(0, 0, 0, 0);
+        if (fp.want_aux) {
+            fp.aux_accum[2 * px] = make_float4(0, 0, 0, 0);
+            fp.aux_accum[2 * px + 1] = make_float4(0, 0, 0, 0);
+        }
+    }
+    if (px == 0) {
+        *fp.batch = 0;
+        fp.totals[0] = fp.totals[1] = fp.totals[2] = 0;
+        for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
+    }
+}
+
+// Averages into the caller's layout: 3 floats a pixel, the band's top row
+// first (renderer.hpp:397-407).
+__global__ __launch_bounds__(256) void k_resolve(FrameParams fp, float* rgb, float* aux) {
+    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= fp.npix) return;
+    const float scale = 1.0f / static_cast<float>(fp.spp);
+    const float4 acc = fp.accum[px];
+    rgb[3 * px] = acc.x * scale;
+    rgb[3 * px + 1] = acc.y * scale;
+    rgb[3 * px + 2] = acc.z * scale;
+    if (aux) {
+        const float4 a0 = fp.aux_accum[2 * px], a1 = fp.aux_accum[2 * px + 1];
+        float* o = aux + 7 * static_cast<size_t>(px);
+        o[0] = a0.x * scale;
+        o[1] = a0.y * scale;
+        o[2] = a0.z * scale;
+        o[3] = a1.x * scale;
+        o[4] = a1.y * scale;
+        o[5] = a1.z * scale;
+        o[6] = a0.w * scale;
+    }
+}
+
+template <int S>
+static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev_e,
+                        hipEvent_t* ev_s) {
+    k_raygen<<<(fp.P + 255) / 256, 256, 0, stream>>>(fp);
+    for (int b = 0; b < fp.depth; ++b) {
+        if (ev_e) (void)hipEventRecord(ev_e[2 * b], stream);
+        if (fp.visits) k_extend<S, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        else k_extend<S, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        if (ev_e) (void)hipEventRecord(ev_e[2 * b + 1], stream);
+        k_shade<<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        if (ev_s) (void)hipEventRecord(ev_s[2 * b], stream);
+        k_shadow<S><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        if (ev_s) (void)hipEventRecord(ev_s[2 * b + 1], stream);
+    }
+    k_accumulate<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp);
+    k_finish<<<1, 64, 0, stream>>>(fp);
+}
+
+hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
+                        hipEvent_t* ev_extend, hipEvent_t* ev_shadow) {
+    if (fp.depth < 1 || fp.depth >= MAX_BOUNCES) return hipErrorInvalidValue;
+    switch (stack_entries) {
+        case 8: launch_loop<8>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow); break;
+        case 16: launch_loop<16>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow); break;
+        case 32: launch_loop<32>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStream_t stream) {
+    k_resolve<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp, rgb, aux);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_frame(const FrameParams& fp, hipStream_t stream) {
+    k_clear_frame<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp);
+    return hipGetLastError();
+}
+
+}  // namespace ptd

@@ -1,0 +1,277 @@
+// Closest-hit and any-hit traversal of the packed scene for one ray per lane.
+//
+// The walk is the reference's (bvh.hpp:57-106, geometry.hpp:111-190,
+// 212-229): large spheres in double first, then the flat tree, then the
+// instance tree, whose leaves move the ray into each instance's space and walk
+// that mesh's tree. Entries are popped nearest first, an entry whose entry
+// distance is not below the current t_max is skipped, hit children are pushed
+// farthest first with ties kept in child order, and leaves test primitives in
+// order, so the primitive found on an exact tie is the reference's.
+//
+// The stack is a short ring in LDS (S entries per lane, lane-interleaved so a
+// wave's accesses spread over all banks) backed by a per-lane spill area in
+// HBM for the deep end: pushing onto a full ring moves its oldest entry out,
+// popping past the ring's bottom reads it back. Entries are 8 bytes: the node
+// or leaf start in the low 28 bits and a leaf's count above, and the entry t.
+#pragma once
+
+#include "device.hpp"
+
+namespace ptd {
+
+template <int S, int BLOCK>
+struct LaneStack {
+    uint2* lds;        // this lane's slot 0; slot k is lds[k * BLOCK]
+    uint2* spill;      // this lane's entry 0 in HBM; entry k is spill[k * stride]
+    uint32_t stride;
+    int top, base;     // entries [0, base) are in HBM, [base, top) in the ring
+
+    __device__ __forceinline__ void reset() { top = base = 0; }
+    __device__ __forceinline__ void push(uint32_t code, float t) {
+        if (top - base == S) {
+            spill[static_cast<size_t>(base) * stride] = lds[(base & (S - 1)) * BLOCK];
+            ++base;
+        }
+        lds[(top & (S - 1)) * BLOCK] = make_uint2(code, __float_as_uint(t));
+        ++top;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        --top;
+        if (top >= base) return lds[(top & (S - 1)) * BLOCK];
+        base = top;
+        return spill[static_cast<size_t>(top) * stride];
+    }
+};
+
+constexpr uint32_t FIRST_MASK = 0x0fffffffu;
+
+// Visit counts of one query, kept only by the counting kernel variants: node
+// slab tests, leaf references, triangle tests, sphere tests, instance
+// entries, large-sphere tests. They price a query's algorithmic bytes.
+struct Visits {
+    uint32_t nodes = 0, leafrefs = 0, tris = 0, spheres = 0, instances = 0, large = 0;
+};
+constexpr int COUNT_SHIFT = 28;
+
+// The slab test of a node's four children, then the hit children pushed
+// farthest first (bvh.hpp:80-104). IEEE min/max stand in for SSE's
+// minps/maxps; they differ only where 0 * inf makes a NaN (a ray with a zero
+// direction component starting exactly on a box face).
+template <class Stack>
+__device__ __forceinline__ void visit_node(const Node* __restrict__ nodes, uint32_t idx, f3 o, f3 inv, float tmin,
+                                           float tmax, Stack& st) {
+    const Node* nd = nodes + idx;
+    const float4 lx = nd->lo[0], ly = nd->lo[1], lz = nd->lo[2];
+    const float4 hx = nd->hi[0], hy = nd->hi[1], hz = nd->hi[2];
+    const uint4 fi = nd->first, co = nd->count;
+    const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
+    const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+    const uint32_t code[4] = {fi.x | (co.x << COUNT_SHIFT), fi.y | (co.y << COUNT_SHIFT), fi.z | (co.z << COUNT_SHIFT),
+                              fi.w | (co.w << COUNT_SHIFT)};
+    float tn[4];
+    bool h[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float x0 = (lxa[c] - o.x) * inv.x, x1 = (hxa[c] - o.x) * inv.x;
+        const float y0 = (lya[c] - o.y) * inv.y, y1 = (hya[c] - o.y) * inv.y;
+        const float z0 = (lza[c] - o.z) * inv.z, z1 = (hza[c] - o.z) * inv.z;
+        const float near = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+        const float far = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+        tn[c] = near;
+        h[c] = near <= far;
+    }
+    // Position of each hit child in the farthest-first order: those entered
+    // later come first, and on equal entry the lower child number first.
+    int rank[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        int r = 0;
+#pragma unroll
+        for (int c2 = 0; c2 < 4; ++c2) {
+            if (c2 == c) continue;
+            const bool before = tn[c2] > tn[c] || (tn[c2] == tn[c] && c2 < c);
+            r += (h[c2] && before) ? 1 : 0;
+        }
+        rank[c] = r;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t cd = 0;
+        float t = 0;
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool sel = h[c] && rank[c] == k;
+            cd = sel ? code[c] : cd;
+            t = sel ? tn[c] : t;
+            any = any || sel;
+        }
+        if (any) st.push(cd, t);
+    }
+}
+
+__device__ __forceinline__ f3 safe_rcp3(f3 d) { return {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
+
+// Returns whether something was hit. With ANY, stops at the first hit
+// (geometry::occluded); otherwise fills `hit` with the nearest.
+template <bool ANY, bool COUNT = false, class Stack>
+__device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float tmin, float tmax, Hit& hit,
+                                          Stack& st, Visits* vis = nullptr) {
+    uint32_t kind = 0, prim = 0, inst = NONE;
+    float hu = 0, hv = 0;
+
+    // Large spheres, in double (geometry.hpp:117-118, 157-160).
+    for (uint32_t k = 0; k < sc.n_large; ++k) {
+        if (COUNT) ++vis->large;
+        const uint32_t s = sc.large[k];
+        if (sphere_hit_d(sc.spheres[2 * s], o, d, tmin, tmax)) {
+            if (ANY) return true;
+            kind = 1;
+            prim = s;
+        }
+    }
+    const f3 inv = safe_rcp3(d);
+
+    // The flat tree: spheres then triangles (geometry.hpp:119-127).
+    if (sc.flat_root != NONE) {
+        st.reset();
+        st.push(sc.flat_root, tmin);
+        while (st.top > 0) {
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) >= tmax) continue;
+            const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
+            if (count == 0) {
+                if (COUNT) ++vis->nodes;
+                visit_node(sc.nodes, first, o, inv, tmin, tmax, st);
+                continue;
+            }
+            for (uint32_t i = first; i < first + count; ++i) {
+                const float4 r0 = sc.leaf[3 * i], r1 = sc.leaf[3 * i + 1];
+                if (COUNT) {
+                    ++vis->leafrefs;
+                    if (__float_as_uint(r1.w) != 0u) ++vis->spheres;
+                    else ++vis->tris;
+                }
+                if (__float_as_uint(r1.w) != 0u) {
+                    const float4 cr = make_float4(r0.x, r0.y, r0.z, r1.x);
+                    if (sphere_hit_f(cr, o, d, tmin, tmax)) {
+                        if (ANY) return true;
+                        kind = 2;
+                        prim = __float_as_uint(r0.w);
+                        inst = NONE;
+                    }
+                } else {
+                    const float4 r2 = sc.leaf[3 * i + 2];
+                    if (tri_hit(xyz(r0), xyz(r1), xyz(r2), o, d, tmin, tmax, hu, hv)) {
+                        if (ANY) return true;
+                        kind = 3;
+                        prim = __float_as_uint(r0.w);
+                        inst = NONE;
+                    }
+                }
+            }
+        }
+    }
+
+    // The instance tree; a leaf walks each instance's mesh tree in object
+    // space, where t is the same (geometry.hpp:212-229).
+    if (sc.inst_root != NONE) {
+        st.reset();
+        st.push(sc.inst_root, tmin);
+        bool in_blas = false;
+        int blas_base = 0;
+        uint32_t li = 0, lend = 0, cur = 0;
+        f3 lo = o, ld = d, linv = inv;
+        auto enter = [&](uint32_t k) {
+            if (COUNT) ++vis->instances;
+            cur = k;
+            const float4 r0 = sc.inst[7 * k + 3], r1 = sc.inst[7 * k + 4], r2 = sc.inst[7 * k + 5];
+            const uint32_t root = __float_as_uint(sc.inst[7 * k + 6].x);
+            lo = mk(r0.x * o.x + r0.y * o.y + r0.z * o.z + r0.w, r1.x * o.x + r1.y * o.y + r1.z * o.z + r1.w,
+                    r2.x * o.x + r2.y * o.y + r2.z * o.z + r2.w);
+            ld = mk(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+                    r2.x * d.x + r2.y * d.y + r2.z * d.z);
+            linv = safe_rcp3(ld);
+            blas_base = st.top;
+            in_blas = true;
+            if (root != NONE) st.push(root, tmin);
+        };
+        while (true) {
+            if (in_blas && st.top == blas_base) {
+                if (++li < lend) {
+                    enter(sc.order[li]);
+                    continue;
+                }
+                in_blas = false;
+            }
+            if (st.top == 0) break;
+            const uint2 e = st.pop();
+            if (__uint_as_float(e.y) >= tmax) continue;
+            const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
+            if (count == 0) {
+                if (COUNT) ++vis->nodes;
+                if (in_blas) visit_node(sc.nodes, first, lo, linv, tmin, tmax, st);
+                else visit_node(sc.nodes, first, o, inv, tmin, tmax, st);
+                continue;
+            }
+            if (COUNT) vis->leafrefs += count;
+            if (!in_blas) {
+                li = first;
+                lend = first + count;
+                enter(sc.order[li]);
+                continue;
+            }
+            for (uint32_t i = first; i < first + count; ++i) {
+                if (COUNT) ++vis->tris;
+                const float4 r0 = sc.leaf[3 * i], r1 = sc.leaf[3 * i + 1], r2 = sc.leaf[3 * i + 2];
+                if (tri_hit(xyz(r0), xyz(r1), xyz(r2), lo, ld, tmin, tmax, hu, hv)) {
+                    if (ANY) return true;
+                    kind = 3;
+                    prim = __float_as_uint(r0.w);
+                    inst = cur;
+                }
+            }
+        }
+    }
+    if (ANY) return false;
+    hit.t = tmax;
+    hit.u = hu;
+    hit.v = hv;
+    hit.kind = kind;
+    hit.prim = prim;
+    hit.inst = inst;
+    return kind != 0;
+}
+
+// One lane's stack over the block's LDS ring array and the grid's spill area.
+template <int S, int BLOCK>
+__device__ __forceinline__ LaneStack<S, BLOCK> lane_stack(uint2* lds, uint2* spill) {
+    LaneStack<S, BLOCK> st;
+    st.lds = lds + threadIdx.x;
+    const uint32_t stride = gridDim.x * BLOCK;
+    st.spill = spill + (blockIdx.x * BLOCK + threadIdx.x);
+    st.stride = stride;
+    st.top = st.base = 0;
+    return st;
+}
+
+// Adds a lane's visit counts into six 64-bit totals, one atomic per wave and counter.
+__device__ __forceinline__ void add_visits(unsigned long long* totals, const Visits& v) {
+    const uint32_t x[6] = {v.nodes, v.leafrefs, v.tris, v.spheres, v.instances, v.large};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        unsigned long long s = x[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if ((threadIdx.x & 63) == 0 && s) atomicAdd(totals + k, s);
+    }
+}
+
+// A wave takes `n` items at a time from a shared counter.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter, uint32_t n) {
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, n);
+    return __shfl(base, 0, 64);
+}
+
+}  // namespace ptd

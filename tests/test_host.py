@@ -1,0 +1,127 @@
+"""Host side of the library, no GPU: the C ABI loads and exports what the
+header declares; scenes pack byte for byte as the reference packs them; bad
+buffers are refused with an error, not a crash."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import photon_tracer_amd as pt
+import render_check as rc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "photon_tracer.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pt_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pt.library()
+    names = header_symbols()
+    assert "pt_render" in names and "pt_trace" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version():
+    assert "gfx950" in pt.version()
+
+
+@pytest.mark.parametrize("name", ["materials", "field", "teapot", "room", "crowd"])
+def test_pack_matches_reference_pack_for_gpu(name):
+    # The fixture holds the reference's own serialize(pack_for_gpu(...)) of
+    # its scene; our builder + packer must reproduce it byte for byte.
+    ptsd, ptgp = rc.load_scene(name)
+    ours = pt.pack(ptsd)
+    assert ours.size == ptgp.size
+    assert np.array_equal(ours, ptgp)
+
+
+@pytest.mark.parametrize("config,scale", [(0, 1.0), (1, 0.01), (2, 0.01), (4, 0.005)])
+def test_generated_scenes_are_deterministic(config, scale):
+    a = pt.generate(config, 5, scale)
+    b = pt.generate(config, 5, scale)
+    c = pt.generate(config, 6, scale)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)
+    assert np.array_equal(pt.pack(a), pt.pack(b))
+
+
+def test_config_sizes():
+    w = pt.generate(0, 0, 1.0)
+    assert w[11] == 22 * 22 + 4 and w[12] == 1          # 484 small + 3 big + 1 light; the ground
+    w = pt.generate(1, 0, 1.0)
+    assert w[13] == 20 * 224 * 224                      # 1,003,520 triangles
+    w = pt.generate(2, 0, 1.0)
+    assert w[14] == 3 and w[15] == 4096 and w[16] == 50_000 + 200_000 + 500_000
+
+
+def test_deep_tree_is_packed():
+    # 1M triangles: 11 levels, deeper than the reference's WebGPU packer allows.
+    ptgp = pt.pack(pt.generate(1, 1, 1.0))
+    assert pt.ptgp_depth(ptgp) > 10
+
+
+def test_job_generator():
+    jobs = [pt.generate_job(s, 0.01) for s in range(8)]
+    assert {j.kind for j in jobs} == {0, 1}
+    for j in jobs:
+        assert (j.width, j.height) in {(256, 256), (512, 512), (1024, 576), (1024, 1024)}
+        assert j.spp in (4, 16, 64)
+        assert pt.pack(j.ptsd).size > 48
+
+
+def test_bad_buffers_are_refused():
+    ptgp = pt.pack(pt.generate(0, 0, 0.25))
+    assert pt.ptgp_depth(ptgp) >= 1
+    bad = ptgp.copy()
+    bad[0] = 0
+    with pytest.raises(pt.PTError, match="magic"):
+        pt.ptgp_depth(bad)
+    bad = ptgp.copy()
+    bad[3] += 64   # nodes section runs past the end
+    with pytest.raises(pt.PTError):
+        pt.ptgp_depth(bad)
+    bad = ptgp.copy()
+    node0 = int(bad[2]) // 4 + int(bad[18]) * 32
+    bad[node0 + 24: node0 + 28] = 0xFFFFFF   # child links out of range
+    bad[node0 + 28: node0 + 32] = 0
+    with pytest.raises(pt.PTError):
+        pt.ptgp_depth(bad)
+    with pytest.raises(pt.PTError):
+        pt.ptgp_depth(ptgp[:20])
+    with pytest.raises(pt.PTError, match="PTSD"):
+        pt.pack(np.zeros(40, np.uint32))
+    with pytest.raises(pt.PTError):
+        pt.generate(9, 0, 1.0)
+
+
+def test_cycle_is_refused():
+    ptgp = pt.pack(pt.generate(0, 0, 0.25))
+    bad = ptgp.copy()
+    root = int(bad[18])
+    node = int(bad[2]) // 4 + root * 32
+    # Point the root's first interior child back at the root.
+    for c in range(4):
+        if bad[node + 28 + c] == 0 and np.frombuffer(bad[node + c:node + c + 1].tobytes(), np.float32)[0] != np.inf:
+            bad[node + 24 + c] = root
+            break
+    with pytest.raises(pt.PTError, match="cycle"):
+        pt.ptgp_depth(bad)
+
+
+def test_pfm_round_trip(tmp_path):
+    img = np.random.default_rng(0).random((5, 7, 3)).astype(np.float32)
+    p = str(tmp_path / "x.pfm")
+    pt.write_pfm(p, img)
+    assert np.array_equal(pt.read_pfm(p), img)
+    raw = open(p, "rb").read()
+    assert raw.startswith(b"PF\n7 5\n-1.0\n")
+    # bottom row first (image_write.hpp:24-28)
+    first = np.frombuffer(raw[len(b"PF\n7 5\n-1.0\n"):][: 7 * 12], "<f4").reshape(7, 3)
+    assert np.array_equal(first, img[-1])
