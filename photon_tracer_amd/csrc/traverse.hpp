@@ -56,10 +56,12 @@ constexpr int COUNT_SHIFT = 28;
 // The slab test of a node's four children, then the hit children pushed
 // farthest first (bvh.hpp:80-104). IEEE min/max stand in for SSE's
 // minps/maxps; they differ only where 0 * inf makes a NaN (a ray with a zero
-// direction component starting exactly on a box face).
+// direction component starting exactly on a box face). The nearest hit child,
+// which the reference pushes last and pops at once, is returned in `next`
+// instead of going through the stack; returns whether there is one.
 template <class Stack>
-__device__ __forceinline__ void visit_node(const Node* __restrict__ nodes, uint32_t idx, f3 o, f3 inv, float tmin,
-                                           float tmax, Stack& st) {
+__device__ __forceinline__ bool visit_node(const Node* __restrict__ nodes, uint32_t idx, f3 o, f3 inv, float tmin,
+                                           float tmax, Stack& st, uint2& next) {
     const Node* nd = nodes + idx;
     const float4 lx = nd->lo[0], ly = nd->lo[1], lz = nd->lo[2];
     const float4 hx = nd->hi[0], hy = nd->hi[1], hz = nd->hi[2];
@@ -94,20 +96,21 @@ __device__ __forceinline__ void visit_node(const Node* __restrict__ nodes, uint3
         }
         rank[c] = r;
     }
+    const int n = (h[0] ? 1 : 0) + (h[1] ? 1 : 0) + (h[2] ? 1 : 0) + (h[3] ? 1 : 0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         uint32_t cd = 0;
         float t = 0;
-        bool any = false;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const bool sel = h[c] && rank[c] == k;
             cd = sel ? code[c] : cd;
             t = sel ? tn[c] : t;
-            any = any || sel;
         }
-        if (any) st.push(cd, t);
+        if (k < n - 1) st.push(cd, t);
+        else if (k == n - 1) next = make_uint2(cd, __float_as_uint(t));
     }
+    return n > 0;
 }
 
 __device__ __forceinline__ f3 safe_rcp3(f3 d) { return {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
@@ -135,14 +138,19 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
     // The flat tree: spheres then triangles (geometry.hpp:119-127).
     if (sc.flat_root != NONE) {
         st.reset();
-        st.push(sc.flat_root, tmin);
-        while (st.top > 0) {
-            const uint2 e = st.pop();
+        uint2 e = make_uint2(sc.flat_root, __float_as_uint(tmin));
+        bool have = true;
+        while (true) {
+            if (!have) {
+                if (st.top == 0) break;
+                e = st.pop();
+            }
+            have = false;
             if (__uint_as_float(e.y) >= tmax) continue;
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                visit_node(sc.nodes, first, o, inv, tmin, tmax, st);
+                have = visit_node(sc.nodes, first, o, inv, tmin, tmax, st, e);
                 continue;
             }
             for (uint32_t i = first; i < first + count; ++i) {
@@ -177,7 +185,8 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
     // space, where t is the same (geometry.hpp:212-229).
     if (sc.inst_root != NONE) {
         st.reset();
-        st.push(sc.inst_root, tmin);
+        uint2 e = make_uint2(sc.inst_root, __float_as_uint(tmin));
+        bool have = true;
         bool in_blas = false;
         int blas_base = 0;
         uint32_t li = 0, lend = 0, cur = 0;
@@ -194,24 +203,28 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
             linv = safe_rcp3(ld);
             blas_base = st.top;
             in_blas = true;
-            if (root != NONE) st.push(root, tmin);
+            e = make_uint2(root, __float_as_uint(tmin));
+            have = root != NONE;
         };
         while (true) {
-            if (in_blas && st.top == blas_base) {
-                if (++li < lend) {
-                    enter(sc.order[li]);
-                    continue;
+            if (!have) {
+                if (in_blas && st.top == blas_base) {
+                    if (++li < lend) {
+                        enter(sc.order[li]);
+                        continue;
+                    }
+                    in_blas = false;
                 }
-                in_blas = false;
+                if (st.top == 0) break;
+                e = st.pop();
             }
-            if (st.top == 0) break;
-            const uint2 e = st.pop();
+            have = false;
             if (__uint_as_float(e.y) >= tmax) continue;
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                if (in_blas) visit_node(sc.nodes, first, lo, linv, tmin, tmax, st);
-                else visit_node(sc.nodes, first, o, inv, tmin, tmax, st);
+                if (in_blas) have = visit_node(sc.nodes, first, lo, linv, tmin, tmax, st, e);
+                else have = visit_node(sc.nodes, first, o, inv, tmin, tmax, st, e);
                 continue;
             }
             if (COUNT) vis->leafrefs += count;
