@@ -158,7 +158,8 @@ enum {
     PT_RENDER_NO_NEE = 1,        /* world.sample_lights = false (main.cpp --no-nee) */
     PT_RENDER_NO_GRAPH = 2,      /* launch the bounce loop eagerly, not as a hipGraph */
     PT_RENDER_TIMING = 4,        /* eager, with HIP events around every traversal launch */
-    PT_RENDER_COUNT = 8          /* eager, closest-hit stage counts its visits (stats->visits) */
+    PT_RENDER_COUNT = 8,         /* eager, closest-hit stage counts its visits (stats->visits) */
+    PT_RENDER_DEVICE_OUTPUT = 16 /* rgb and aux are device addresses (pt_device_alloc): no copy to the host */
 };
 
 /* Render statistics. */
@@ -171,6 +172,8 @@ typedef struct {
     double shadow_ms;          /* with PT_RENDER_TIMING: time in any-hit traversal */
     uint64_t visits[6];        /* with PT_RENDER_COUNT: closest-hit node, leaf-ref,
                                   triangle, sphere, instance, large-sphere visits */
+    int32_t batches;           /* batches the samples were split into */
+    int32_t samples_per_batch; /* samples per pixel in each batch */
 } pt_render_stats;
 
 /* rgb: 3 floats a pixel, top row first, for the rows rendered (the band's top
@@ -178,6 +181,13 @@ typedef struct {
  * renderer.hpp:358-362). stats: optional. */
 pt_status pt_render(pt_scene* scene, const pt_render_params* params, float* rgb, float* aux,
                     pt_render_stats* stats);
+
+/* Device memory for callers without the HIP headers (ray and hit buffers for
+ * pt_trace_device, images for PT_RENDER_DEVICE_OUTPUT). No counterpart in the
+ * reference, which has no device. */
+pt_status pt_device_alloc(size_t bytes, void** ptr);
+pt_status pt_device_free(void* ptr);
+pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int to_device);
 
 #ifdef __cplusplus
 }

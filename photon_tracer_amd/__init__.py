@@ -33,6 +33,7 @@ RENDER_NO_NEE = 1
 RENDER_NO_GRAPH = 2
 RENDER_TIMING = 4
 RENDER_COUNT = 8
+RENDER_DEVICE_OUTPUT = 16
 
 # Algorithmic bytes of one traversal query (the ray in, the hit out, and every
 # record the reference's walk must read): 32 + 16 per query, 128 per node slab
@@ -63,7 +64,7 @@ class _RenderStats(ctypes.Structure):
     _fields_ = [
         ("rays", ctypes.c_uint64), ("primary_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
         ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
-        ("visits", ctypes.c_uint64 * 6),
+        ("visits", ctypes.c_uint64 * 6), ("batches", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
     ]
 
 
@@ -104,7 +105,10 @@ def library() -> ctypes.CDLL:
     lib.pt_trace_device.argtypes = [vp, vp, sz, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_float),
                                     ctypes.POINTER(ctypes.c_uint64)]
     lib.pt_render.argtypes = [vp, ctypes.POINTER(_RenderParams), vp, vp, ctypes.POINTER(_RenderStats)]
-    for name in ("pt_device_info", "pt_generate", "pt_generate_job", "pt_pack", "pt_ptgp_depth", "pt_scene_create",
+    lib.pt_device_alloc.argtypes = [sz, ctypes.POINTER(vp)]
+    lib.pt_device_free.argtypes = [vp]
+    lib.pt_device_copy.argtypes = [vp, vp, sz, ctypes.c_int]
+    for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate", "pt_generate_job", "pt_pack", "pt_ptgp_depth", "pt_scene_create",
                  "pt_scene_destroy", "pt_trace", "pt_trace_device", "pt_render"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
@@ -192,6 +196,8 @@ class RenderStats:
     extend_ms: float
     shadow_ms: float
     visits: tuple = (0, 0, 0, 0, 0, 0)
+    batches: int = 0
+    samples_per_batch: int = 0
 
     @property
     def mrays_per_s(self) -> float:
@@ -259,23 +265,62 @@ class Scene:
     def render(self, width: int, height: int, spp: int, max_bounces: int = 10, seed: int = 0,
                first_sample: int = 0, rows: tuple[int, int] | None = None, nee: bool = True, aux: bool = False,
                graph: bool = True, timing: bool = False, count: bool = False, samples_per_batch: int = 0,
-               stack_entries: int = 0):
+               stack_entries: int = 0, device_out: "DeviceBuffer | None" = None):
         """Renders rows [j0, j1) (from the bottom; default all). Returns (rgb, aux, stats) with rgb of shape
         (rows, width, 3), the band's top row first, as render_row_linear writes rows."""
         j0, j1 = rows if rows else (0, height)
         p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, j0, j1,
                           (0 if nee else RENDER_NO_NEE) | (0 if graph else RENDER_NO_GRAPH) |
-                          (RENDER_TIMING if timing else 0) | (RENDER_COUNT if count else 0), samples_per_batch,
+                          (RENDER_TIMING if timing else 0) | (RENDER_COUNT if count else 0) |
+                          (RENDER_DEVICE_OUTPUT if device_out is not None else 0), samples_per_batch,
                           stack_entries)
         nrows = j1 - j0
-        rgb = np.zeros((max(nrows, 0), width, 3), np.float32)
-        ax = np.zeros((max(nrows, 0), width, 7), np.float32) if aux else None
         st = _RenderStats()
-        _check(library().pt_render(self._h, ctypes.byref(p), rgb.ctypes.data, ax.ctypes.data if aux else None,
-                                   ctypes.byref(st)))
+        if device_out is not None:
+            # The image stays in HBM, in device_out; no aux in this mode.
+            assert device_out.nbytes >= 12 * max(nrows, 0) * width and not aux
+            _check(library().pt_render(self._h, ctypes.byref(p), ctypes.c_void_p(device_out.ptr), None,
+                                       ctypes.byref(st)))
+            rgb, ax = None, None
+        else:
+            rgb = np.zeros((max(nrows, 0), width, 3), np.float32)
+            ax = np.zeros((max(nrows, 0), width, 7), np.float32) if aux else None
+            _check(library().pt_render(self._h, ctypes.byref(p), rgb.ctypes.data, ax.ctypes.data if aux else None,
+                                       ctypes.byref(st)))
         stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
-                            tuple(st.visits))
+                            tuple(st.visits), st.batches, st.samples_per_batch)
         return rgb, ax, stats
+
+
+class DeviceBuffer:
+    """A block of HBM owned by the library's allocator (pt_device_alloc)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(library().pt_device_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(library().pt_device_copy(ctypes.c_void_p(self.ptr), a.ctypes.data, a.nbytes, 1))
+
+    def download(self, a: np.ndarray) -> np.ndarray:
+        assert a.flags.c_contiguous and a.nbytes <= self.nbytes
+        _check(library().pt_device_copy(a.ctypes.data, ctypes.c_void_p(self.ptr), a.nbytes, 0))
+        return a
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(library().pt_device_free(ctypes.c_void_p(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def write_pfm(path: str, rgb: np.ndarray) -> None:

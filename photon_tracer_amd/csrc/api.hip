@@ -44,16 +44,26 @@ struct dbuf {
     }
 };
 
-constexpr size_t kTargetPaths = size_t(1) << 23;  // paths in flight per batch
+// Paths in flight per batch: up to 2^27 (a 1080p frame at 64 spp in one
+// batch), or what a quarter of the free HBM holds at kPathBytes each.
+constexpr size_t kMaxPaths = size_t(1) << 27;
+constexpr size_t kPathBytes = 2 * 36 + 24 + 48 + 48;
 constexpr int kDefaultStack = 16;
 
 struct render_bufs {
     dbuf ro[2], rd[2], rpath[2], hits, so, sd, sl, pa, pb, pkey, pq, paux;
-    dbuf accum, aux_accum, ctrl, batch, totals, out_rgb, out_aux;
+    dbuf accum, aux_accum, ctrl, frame, totals, out_rgb, out_aux;
+    std::vector<dbuf*> all() {
+        return {&ro[0], &ro[1], &rd[0], &rd[1], &rpath[0], &rpath[1], &hits, &so, &sd, &sl, &pa, &pb, &pkey, &pq,
+                &paux, &accum, &aux_accum, &ctrl, &frame, &totals, &out_rgb, &out_aux};
+    }
     void release() {
-        for (dbuf* b : {&ro[0], &ro[1], &rd[0], &rd[1], &rpath[0], &rpath[1], &hits, &so, &sd, &sl, &pa, &pb, &pkey, &pq,
-                        &paux, &accum, &aux_accum, &ctrl, &batch, &totals, &out_rgb, &out_aux})
-            b->release();
+        for (dbuf* b : all()) b->release();
+    }
+    size_t bytes() {
+        size_t n = 0;
+        for (dbuf* b : all()) n += b->bytes;
+        return n;
     }
 };
 
@@ -69,7 +79,19 @@ struct pt_scene {
     int max_stack = 0;   // entries a lane's stack can reach in this scene
     int tuned_stack = 0, tuned_blocks = 0;
     render_bufs rb;
+    // The last captured bounce-loop graph and the launch it was captured for.
+    struct graph_cache {
+        hipGraphExec_t exec = nullptr;
+        ptd::FrameParams key{};
+        int S = 0, trace_grid = 0, shade_grid = 0;
+    } graph;
+    void drop_graph() {
+        if (graph.exec) (void)hipGraphExecDestroy(graph.exec);
+        graph.exec = nullptr;
+    }
     ~pt_scene() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
         rb.release();
         for (dbuf* b : {&words, &leaf, &spill, &work, &io_rays, &io_hits, &counts}) b->release();
         if (stream) (void)hipStreamDestroy(stream);
@@ -248,12 +270,13 @@ struct ptgp_check {
     }
 };
 
-int choose_spb(int spp, size_t npix, int requested) {
+int choose_spb(int spp, size_t npix, int requested, size_t free_bytes) {
     if (requested > 0) {
         if (spp % requested) throw pt::status_error(PT_ERR_INVALID, "samples_per_batch must divide spp");
         return requested;
     }
-    const size_t cap = std::max<size_t>(1, kTargetPaths / std::max<size_t>(1, npix));
+    const size_t target = std::min(kMaxPaths, free_bytes / 4 / kPathBytes);
+    const size_t cap = std::max<size_t>(1, target / std::max<size_t>(1, npix));
     int best = 1;
     for (int d = 1; d <= spp; ++d)
         if (spp % d == 0 && size_t(d) <= cap) best = d;
@@ -444,7 +467,10 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             throw pt::status_error(PT_ERR_INVALID, "pt_render: row range outside the image");
         if (P.first_sample < 0) throw pt::status_error(PT_ERR_INVALID, "pt_render: first_sample must be >= 0");
         const size_t npix = size_t(P.width) * size_t(j_hi - j_lo);
-        const int spb = choose_spb(P.spp, npix, P.samples_per_batch);
+        size_t free_bytes = 0, total_bytes = 0;
+        HIPC(hipMemGetInfo(&free_bytes, &total_bytes));
+        free_bytes += s->rb.bytes();  // buffers this scene already holds are reused
+        const int spb = choose_spb(P.spp, npix, P.samples_per_batch, free_bytes);
         const size_t paths = npix * size_t(spb);
         if (paths >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render: batch too large");
         const int S = stack_entries_for(s, P.stack_entries);
@@ -474,7 +500,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         }
         B.accum.ensure(npix * 16);
         B.ctrl.ensure(sizeof(ptd::Ctrl) * (ptd::MAX_BOUNCES + 1));
-        B.batch.ensure(16);
+        B.frame.ensure(sizeof(ptd::FrameConst));
         B.totals.ensure(32);
         B.out_rgb.ensure(npix * 12);
         HIPC(hipMemsetAsync(B.ctrl.p, 0, sizeof(ptd::Ctrl) * (ptd::MAX_BOUNCES + 1), s->stream));
@@ -488,8 +514,6 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         fp.depth = P.max_bounces;
         fp.j_lo = j_lo;
         fp.j_hi = j_hi;
-        fp.seed = P.seed;
-        fp.first_sample = P.first_sample;
         fp.spb = spb;
         fp.P = static_cast<uint32_t>(paths);
         fp.npix = static_cast<uint32_t>(npix);
@@ -512,7 +536,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         fp.accum = B.accum.as<float4>();
         fp.aux_accum = want_aux ? B.aux_accum.as<float4>() : nullptr;
         fp.ctrl = B.ctrl.as<ptd::Ctrl>();
-        fp.batch = B.batch.as<uint32_t>();
+        fp.frame = B.frame.as<ptd::FrameConst>();
         fp.totals = B.totals.as<unsigned long long>();
         fp.spill = s->spill.as<uint2>();
         const bool counting = stats && (P.flags & PT_RENDER_COUNT);
@@ -526,6 +550,8 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         hipEvent_t e0, e1;
         HIPC(hipEventCreate(&e0));
         HIPC(hipEventCreate(&e1));
+        const ptd::FrameConst fc{P.seed, P.first_sample, 0u};
+        HIPC(hipMemcpyAsync(B.frame.p, &fc, sizeof fc, hipMemcpyHostToDevice, s->stream));
         HIPC(ptd::launch_clear_frame(fp, s->stream));
         HIPC(hipEventRecord(e0, s->stream));
         double extend_ms = 0, shadow_ms = 0;
@@ -550,27 +576,30 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             for (int b = 0; b < batches; ++b)
                 HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr, nullptr));
         } else {
-            // The batch's whole bounce loop as one graph, replayed per batch;
-            // kernels read queue sizes and the batch number from HBM.
-            hipGraph_t graph;
-            hipGraphExec_t exec;
-            HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t le = ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr, nullptr);
-            const hipError_t ce = hipStreamEndCapture(s->stream, &graph);
-            HIPC(le);
-            HIPC(ce);
-            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            HIPC(ie);
-            for (int b = 0; b < batches; ++b) {
-                const hipError_t ge = hipGraphLaunch(exec, s->stream);
-                if (ge != hipSuccess) {
-                    (void)hipGraphExecDestroy(exec);
-                    HIPC(ge);
-                }
+            // The batch's whole bounce loop as one graph, replayed per batch
+            // and kept for the next render of the same shape; kernels read
+            // queue sizes, the seed and the batch number from HBM.
+            auto& g = s->graph;
+            const bool reuse = g.exec && g.S == S && g.trace_grid == trace_grid && g.shade_grid == shade_grid &&
+                               std::memcmp(&g.key, &fp, sizeof fp) == 0;
+            if (!reuse) {
+                s->drop_graph();
+                hipGraph_t graph;
+                HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                const hipError_t le = ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr, nullptr);
+                const hipError_t ce = hipStreamEndCapture(s->stream, &graph);
+                HIPC(le);
+                HIPC(ce);
+                const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) g.exec = nullptr;
+                HIPC(ie);
+                g.key = fp;
+                g.S = S;
+                g.trace_grid = trace_grid;
+                g.shade_grid = shade_grid;
             }
-            HIPC(hipStreamSynchronize(s->stream));
-            (void)hipGraphExecDestroy(exec);
+            for (int b = 0; b < batches; ++b) HIPC(hipGraphLaunch(g.exec, s->stream));
         }
         HIPC(hipEventRecord(e1, s->stream));
         HIPC(hipEventSynchronize(e1));
@@ -579,9 +608,14 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
 
-        HIPC(ptd::launch_resolve(fp, B.out_rgb.as<float>(), want_aux ? B.out_aux.as<float>() : nullptr, s->stream));
-        HIPC(hipMemcpyAsync(rgb, B.out_rgb.p, npix * 12, hipMemcpyDeviceToHost, s->stream));
-        if (want_aux) HIPC(hipMemcpyAsync(aux, B.out_aux.p, npix * 28, hipMemcpyDeviceToHost, s->stream));
+        if (P.flags & PT_RENDER_DEVICE_OUTPUT) {
+            // rgb and aux are device addresses: the image stays in HBM.
+            HIPC(ptd::launch_resolve(fp, rgb, aux, s->stream));
+        } else {
+            HIPC(ptd::launch_resolve(fp, B.out_rgb.as<float>(), want_aux ? B.out_aux.as<float>() : nullptr, s->stream));
+            HIPC(hipMemcpyAsync(rgb, B.out_rgb.p, npix * 12, hipMemcpyDeviceToHost, s->stream));
+            if (want_aux) HIPC(hipMemcpyAsync(aux, B.out_aux.p, npix * 28, hipMemcpyDeviceToHost, s->stream));
+        }
         unsigned long long totals[3] = {0, 0, 0};
         HIPC(hipMemcpyAsync(totals, B.totals.p, sizeof totals, hipMemcpyDeviceToHost, s->stream));
         HIPC(hipStreamSynchronize(s->stream));
@@ -595,8 +629,23 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
             if (counting) HIPC(hipMemcpy(v, s->counts.p, 48, hipMemcpyDeviceToHost));
             for (int k = 0; k < 6; ++k) stats->visits[k] = v[k];
+            stats->batches = batches;
+            stats->samples_per_batch = spb;
         }
     });
+}
+
+pt_status pt_device_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return pt::fail(PT_ERR_INVALID, "pt_device_alloc: null output");
+    *ptr = nullptr;
+    PT_GUARD(HIPC(hipMalloc(ptr, bytes ? bytes : 16)));
+}
+
+pt_status pt_device_free(void* ptr) { PT_GUARD(if (ptr) HIPC(hipFree(ptr))); }
+
+pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int to_device) {
+    if ((!dst || !src) && bytes) return pt::fail(PT_ERR_INVALID, "pt_device_copy: null pointer");
+    PT_GUARD(if (bytes) HIPC(hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost)));
 }
 
 }  // extern "C"

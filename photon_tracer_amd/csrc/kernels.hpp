@@ -21,12 +21,19 @@ struct Ctrl {
 };
 constexpr int MAX_BOUNCES = 64;
 
+// Per-render values the kernels read from HBM rather than from their
+// arguments, so that one captured graph serves every render of the same shape.
+struct FrameConst {
+    uint64_t seed;
+    int32_t first_sample;
+    uint32_t batch;       // batch number, advanced by k_finish
+};
+
 struct FrameParams {
     SceneView sc;
     int W, H, spp, depth;
     int j_lo, j_hi;           // rows rendered: j in [j_lo, j_hi), counted from the bottom
-    uint64_t seed;
-    int first_sample, spb;    // samples per pixel per batch
+    int spb;                  // samples per pixel per batch
     uint32_t P, npix;         // paths per batch, pixels in the band
     int nee, want_aux;
     // ray queues (double buffered by bounce parity): (o, tmin), (d, tmax), path
@@ -47,7 +54,7 @@ struct FrameParams {
     float4* accum;            // per pixel: rgb sum
     float4* aux_accum;        // 2 per pixel: (albedo sum, lum^2 sum), (normal sum, 0)
     Ctrl* ctrl;               // MAX_BOUNCES + 1 entries
-    uint32_t* batch;          // batch number, advanced by k_finish
+    FrameConst* frame;        // seed, first sample, batch number
     unsigned long long* totals;  // [0] closest-hit rays, [1] shadow rays, [2] primary rays
     uint2* spill;             // traversal stack spill area
     unsigned long long* visits;  // null, or six totals the closest-hit stage counts into

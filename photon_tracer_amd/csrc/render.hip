@@ -69,13 +69,14 @@ __device__ __forceinline__ float schlick(float cosine, float ratio) {
 __global__ __launch_bounds__(256) void k_raygen(FrameParams fp) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= fp.P) return;
-    const uint32_t batch = *fp.batch;
+    const FrameConst fc = *fp.frame;
+    const uint32_t batch = fc.batch;
     const uint32_t pixel = p / fp.spb, s = p % fp.spb;
     const int i = static_cast<int>(pixel % fp.W);
     const int k = static_cast<int>(pixel / fp.W);   // row within the band, top first
     const int j = fp.j_hi - 1 - k;
-    const uint32_t index = static_cast<uint32_t>(fp.first_sample) + batch * fp.spb + s;
-    const uint32_t pseed = pixel_seed(fp.seed, i, j);
+    const uint32_t index = static_cast<uint32_t>(fc.first_sample) + batch * fp.spb + s;
+    const uint32_t pseed = pixel_seed(fc.seed, i, j);
     float du, dv;
     qmc_pair(pseed, index, 0, du, dv);
     // renderer.hpp:383-385, camera.hpp:45-47
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams fp) {
     fp.rpath[0][p] = p;
     fp.pa[p] = make_float4(1, 1, 1, __uint_as_float(0u));
     fp.pb[p] = make_float4(0, 0, 0, __uint_as_float(0u));
-    const uint64_t key = path_key(fp.seed, i, j, index);
+    const uint64_t key = path_key(fc.seed, i, j, index);
     fp.pkey[p] = make_uint2(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32));
     fp.pq[p] = make_uint2(pseed, index);
     if (p == 0) fp.ctrl[0].n_rays = fp.P;
@@ -401,7 +402,7 @@ __global__ void k_finish(FrameParams fp) {
     fp.totals[1] += shadow;
     fp.totals[2] += fp.ctrl[0].n_rays;
     for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
-    *fp.batch += 1;
+    fp.frame->batch += 1;
 }
 
 __global__ __launch_bounds__(256) void k_clear_frame(FrameParams fp) {
@@ -414,7 +415,7 @@ __global__ __launch_bounds__(256) void k_clear_frame(FrameParams fp) {
         }
     }
     if (px == 0) {
-        *fp.batch = 0;
+        fp.frame->batch = 0;
         fp.totals[0] = fp.totals[1] = fp.totals[2] = 0;
         for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
     }
