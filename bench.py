@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the library's choice)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -116,14 +117,14 @@ def main():
     # from the start); the host copy is timed separately below.
     image = pt.DeviceBuffer(W * H * 12)
     for k in range(args.warmup):
-        _, _, st = scene.render(W, H, spp, seed=1000 + k, device_out=image)
+        _, _, st = scene.render(W, H, spp, seed=1000 + k, device_out=image, stack_entries=args.stack)
         log(f"warmup {k}: {st.render_ms:.1f} ms, {st.mrays_per_s:.0f} Mrays/s")
 
     rays = 0
     gpu_ms = 0.0
     t_start = time.perf_counter()
     for k in range(args.steps):
-        _, _, st = scene.render(W, H, spp, seed=1 + k, device_out=image)
+        _, _, st = scene.render(W, H, spp, seed=1 + k, device_out=image, stack_entries=args.stack)
         rays += st.rays
         gpu_ms += st.render_ms
         log(f"step {k}: {st.rays} rays ({st.primary_rays} primary, {st.shadow_rays} shadow), "
@@ -134,7 +135,7 @@ def main():
     log(f"timed: {args.steps} frames in {wall:.3f} s wall ({gpu_ms:.1f} ms device): {value:.1f} Mrays/s, "
         f"{samples / wall / 1e6:.1f} Msamples/s")
     t_h = time.perf_counter()
-    _, _, hst = scene.render(W, H, spp, seed=1)
+    _, _, hst = scene.render(W, H, spp, seed=1, stack_entries=args.stack)
     host_wall = time.perf_counter() - t_h
     log(f"with the image copied to host memory: {hst.rays / host_wall / 1e6:.1f} Mrays/s ({host_wall * 1e3:.1f} ms)")
 
@@ -142,8 +143,8 @@ def main():
     if not args.no_roofline:
         # The same rays as step 0 (seed 1): one render counting the closest-hit
         # stage's visits, one timing each of its launches with HIP events.
-        _, _, cst = scene.render(W, H, spp, seed=1, count=True)
-        _, _, tst = scene.render(W, H, spp, seed=1, timing=True)
+        _, _, cst = scene.render(W, H, spp, seed=1, count=True, stack_entries=args.stack)
+        _, _, tst = scene.render(W, H, spp, seed=1, timing=True, stack_entries=args.stack)
         queries = cst.rays - cst.shadow_rays
         algo = pt.QUERY_BYTES * queries + sum(v * b for v, b in zip(cst.visits, pt.VISIT_BYTES))
         launches = 10 * tst.batches
@@ -151,7 +152,7 @@ def main():
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_extend<16, false> (closest-hit traversal stage)",
+            "kernel": "k_extend (closest-hit traversal stage)",
             "launches": launches, "avg_launch_ms": round(tst.extend_ms / launches, 4),
             "algorithmic_bytes_per_launch": int(algo / launches),
             "per_query": {"queries": queries, "nodes": cst.visits[0] / queries, "leafrefs": cst.visits[1] / queries,

@@ -48,7 +48,6 @@ struct dbuf {
 // batch), or what a quarter of the free HBM holds at kPathBytes each.
 constexpr size_t kMaxPaths = size_t(1) << 27;
 constexpr size_t kPathBytes = 2 * 36 + 24 + 48 + 48;
-constexpr int kDefaultStack = 16;
 
 struct render_bufs {
     dbuf ro[2], rd[2], rpath[2], hits, so, sd, sl, pa, pb, pkey, pq, paux;
@@ -284,7 +283,10 @@ int choose_spb(int spp, size_t npix, int requested, size_t free_bytes) {
 }
 
 int stack_entries_for(pt_scene* s, int requested) {
-    int S = requested > 0 ? requested : (s->tuned_stack > 0 ? s->tuned_stack : kDefaultStack);
+    // Without instances the 8-entry ring runs at 8 waves per SIMD; with them
+    // two trees are live at once and 16 entries spill less.
+    const int fallback = s->view.inst_root == NONE ? 8 : 16;
+    int S = requested > 0 ? requested : (s->tuned_stack > 0 ? s->tuned_stack : fallback);
     if (S != 8 && S != 16 && S != 32) throw pt::status_error(PT_ERR_INVALID, "stack_entries must be 8, 16 or 32");
     return S;
 }

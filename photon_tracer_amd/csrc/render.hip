@@ -95,8 +95,9 @@ __global__ __launch_bounds__(256) void k_raygen(FrameParams fp) {
     if (p == 0) fp.ctrl[0].n_rays = fp.P;
 }
 
-template <int S, bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(FrameParams fp, int b) {
+template <int S, bool COUNT, bool INST>
+__global__ __launch_bounds__(TRACE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_extend(FrameParams fp, int b) {
     __shared__ uint2 lds[S * TRACE_BLOCK];
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
     Visits vis;
@@ -111,15 +112,16 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(FrameParams fp, int b) {
         if (q < n) {
             const float4 a = ro[q], bb = rd[q];
             Hit h;
-            trace_ray<false, COUNT>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st, &vis);
+            trace_ray<false, COUNT, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st, &vis);
             fp.hits[q] = h;
         }
     }
     if (COUNT) add_visits(fp.visits, vis);
 }
 
-template <int S>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(FrameParams fp, int b) {
+template <int S, bool INST>
+__global__ __launch_bounds__(TRACE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_shadow(FrameParams fp, int b) {
     __shared__ uint2 lds[S * TRACE_BLOCK];
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
     Ctrl* c = fp.ctrl + b;
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(FrameParams fp, int b) {
         if (q < n) {
             const float4 a = fp.so[q], bb = fp.sd[q];
             Hit h;
-            if (!trace_ray<true>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
+            if (!trace_ray<true, false, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
                 const float4 l = fp.sl[q];
                 const uint32_t p = __float_as_uint(l.w);
                 float4 r = fp.pb[p];
@@ -444,18 +446,18 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams fp, float* rgb, flo
     }
 }
 
-template <int S>
+template <int S, bool INST>
 static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev_e,
                         hipEvent_t* ev_s) {
     k_raygen<<<(fp.P + 255) / 256, 256, 0, stream>>>(fp);
     for (int b = 0; b < fp.depth; ++b) {
         if (ev_e) (void)hipEventRecord(ev_e[2 * b], stream);
-        if (fp.visits) k_extend<S, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
-        else k_extend<S, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        if (fp.visits) k_extend<S, true, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        else k_extend<S, false, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
         if (ev_e) (void)hipEventRecord(ev_e[2 * b + 1], stream);
         k_shade<<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
         if (ev_s) (void)hipEventRecord(ev_s[2 * b], stream);
-        k_shadow<S><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        k_shadow<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
         if (ev_s) (void)hipEventRecord(ev_s[2 * b + 1], stream);
     }
     k_accumulate<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp);
@@ -465,10 +467,20 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
 hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
                         hipEvent_t* ev_extend, hipEvent_t* ev_shadow) {
     if (fp.depth < 1 || fp.depth >= MAX_BOUNCES) return hipErrorInvalidValue;
+    const bool inst = fp.sc.inst_root != NONE;
     switch (stack_entries) {
-        case 8: launch_loop<8>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow); break;
-        case 16: launch_loop<16>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow); break;
-        case 32: launch_loop<32>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow); break;
+        case 8:
+            if (inst) launch_loop<8, true>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            else launch_loop<8, false>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            break;
+        case 16:
+            if (inst) launch_loop<16, true>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            else launch_loop<16, false>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            break;
+        case 32:
+            if (inst) launch_loop<32, true>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            else launch_loop<32, false>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -5,8 +5,9 @@
 
 namespace ptd {
 
-template <int S, bool ANY, bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(SceneView sc, const float4* __restrict__ rays, uint32_t n,
+template <int S, bool ANY, bool COUNT, bool INST>
+__global__ __launch_bounds__(TRACE_BLOCK)
+__attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_trace(SceneView sc, const float4* __restrict__ rays, uint32_t n,
                                                         Hit* __restrict__ hits, uint2* spill, uint32_t* work,
                                                         unsigned long long* counts) {
     __shared__ uint2 lds[S * TRACE_BLOCK];
@@ -21,14 +22,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(SceneView sc, const float
             const float4 a = rays[2 * q], b = rays[2 * q + 1];
             Hit h;
             if (ANY) {
-                const bool blocked = trace_ray<true, COUNT>(sc, xyz(a), xyz(b), a.w, b.w, h, st, &vis);
+                const bool blocked = trace_ray<true, COUNT, INST>(sc, xyz(a), xyz(b), a.w, b.w, h, st, &vis);
                 h.t = 0;
                 h.u = h.v = 0;
                 h.kind = blocked ? 1u : 0u;
                 h.prim = 0;
                 h.inst = NONE;
             } else {
-                trace_ray<false, COUNT>(sc, xyz(a), xyz(b), a.w, b.w, h, st, &vis);
+                trace_ray<false, COUNT, INST>(sc, xyz(a), xyz(b), a.w, b.w, h, st, &vis);
             }
             hits[q] = h;
         }
@@ -36,16 +37,23 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(SceneView sc, const float
     if (COUNT) add_visits(counts, vis);
 }
 
+template <int S, bool INST>
+static void launch_si(const SceneView& sc, const float4* rays, uint32_t n, Hit* hits, bool any_hit, uint2* spill,
+                      uint32_t* work, unsigned long long* counts, int grid, hipStream_t stream) {
+    if (counts) {
+        if (any_hit) k_trace<S, true, true, INST><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
+        else k_trace<S, false, true, INST><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
+    } else {
+        if (any_hit) k_trace<S, true, false, INST><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
+        else k_trace<S, false, false, INST><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
+    }
+}
+
 template <int S>
 static hipError_t launch_s(const SceneView& sc, const float4* rays, uint32_t n, Hit* hits, bool any_hit, uint2* spill,
                            uint32_t* work, unsigned long long* counts, int grid, hipStream_t stream) {
-    if (counts) {
-        if (any_hit) k_trace<S, true, true><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
-        else k_trace<S, false, true><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
-    } else {
-        if (any_hit) k_trace<S, true, false><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
-        else k_trace<S, false, false><<<grid, TRACE_BLOCK, 0, stream>>>(sc, rays, n, hits, spill, work, counts);
-    }
+    if (sc.inst_root != NONE) launch_si<S, true>(sc, rays, n, hits, any_hit, spill, work, counts, grid, stream);
+    else launch_si<S, false>(sc, rays, n, hits, any_hit, spill, work, counts, grid, stream);
     return hipGetLastError();
 }
 
@@ -63,9 +71,9 @@ int trace_blocks_per_cu(int stack_entries) {
     int blocks = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (stack_entries) {
-        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trace<8, false, false>, TRACE_BLOCK, 0); break;
-        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trace<16, false, false>, TRACE_BLOCK, 0); break;
-        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trace<32, false, false>, TRACE_BLOCK, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trace<8, false, false, false>, TRACE_BLOCK, 0); break;
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trace<16, false, false, false>, TRACE_BLOCK, 0); break;
+        case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_trace<32, false, false, false>, TRACE_BLOCK, 0); break;
         default: break;
     }
     return e == hipSuccess && blocks > 0 ? blocks : 1;

@@ -19,31 +19,58 @@
 
 namespace ptd {
 
+// Explicit address spaces, so the ring is read with ds_read and the spill
+// area with global loads (a generic pointer selected between the two becomes
+// a flat load, which waits on every outstanding memory operation).
+// Entries are kept as 64-bit words: the node code low, the entry t high.
+using lds_u64 = __attribute__((address_space(3))) unsigned long long;
+using gbl_u64 = __attribute__((address_space(1))) unsigned long long;
+
+__device__ __forceinline__ unsigned long long pack_entry(uint32_t code, float t) {
+    return (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | code;
+}
+__device__ __forceinline__ uint2 unpack_entry(unsigned long long e) {
+    return make_uint2(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32));
+}
+
 template <int S, int BLOCK>
 struct LaneStack {
-    uint2* lds;        // this lane's slot 0; slot k is lds[k * BLOCK]
-    uint2* spill;      // this lane's entry 0 in HBM; entry k is spill[k * stride]
+    lds_u64* lds;      // this lane's slot 0; slot k is lds[k * BLOCK]
+    gbl_u64* spill;    // this lane's entry 0 in HBM; entry k is spill[k * stride]
     uint32_t stride;
     int top, base;     // entries [0, base) are in HBM, [base, top) in the ring
 
     __device__ __forceinline__ void reset() { top = base = 0; }
     __device__ __forceinline__ void push(uint32_t code, float t) {
         if (top - base == S) {
-            spill[static_cast<size_t>(base) * stride] = lds[(base & (S - 1)) * BLOCK];
+            __builtin_nontemporal_store(lds[(base & (S - 1)) * BLOCK], spill + static_cast<size_t>(base) * stride);
             ++base;
         }
-        lds[(top & (S - 1)) * BLOCK] = make_uint2(code, __float_as_uint(t));
+        lds[(top & (S - 1)) * BLOCK] = pack_entry(code, t);
         ++top;
     }
     __device__ __forceinline__ uint2 pop() {
         --top;
-        if (top >= base) return lds[(top & (S - 1)) * BLOCK];
-        base = top;
-        return spill[static_cast<size_t>(top) * stride];
+        unsigned long long v = lds[(top & (S - 1)) * BLOCK];   // the ring slot, read unconditionally
+        if (top < base) {                                      // below the ring: the entry was spilled
+            base = top;
+            // A load of its own kind, so the compiler cannot merge it with
+            // the ring read into one flat load.
+            v = __builtin_nontemporal_load(spill + static_cast<size_t>(top) * stride);
+        }
+        return unpack_entry(v);
     }
 };
 
 constexpr uint32_t FIRST_MASK = 0x0fffffffu;
+
+// Waves per SIMD the traversal kernels are compiled for. The 8-entry ring
+// without the instance walk fits 64 VGPRs (8 waves, 16 KB of LDS per block
+// of 4 waves) with no scratch; the others keep the compiler's own budget.
+template <int S, bool INST>
+struct trace_occupancy {
+    static constexpr int waves = (S == 8 && !INST) ? 8 : 1;
+};
 
 // Visit counts of one query, kept only by the counting kernel variants: node
 // slab tests, leaf references, triangle tests, sphere tests, instance
@@ -117,7 +144,9 @@ __device__ __forceinline__ f3 safe_rcp3(f3 d) { return {1.0f / d.x, 1.0f / d.y, 
 
 // Returns whether something was hit. With ANY, stops at the first hit
 // (geometry::occluded); otherwise fills `hit` with the nearest.
-template <bool ANY, bool COUNT = false, class Stack>
+// INST = false is the variant for scenes without instances: the instance
+// walk, and the registers it keeps live, are compiled out.
+template <bool ANY, bool COUNT = false, bool INST = true, class Stack>
 __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float tmin, float tmax, Hit& hit,
                                           Stack& st, Visits* vis = nullptr) {
     uint32_t kind = 0, prim = 0, inst = NONE;
@@ -183,7 +212,7 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
 
     // The instance tree; a leaf walks each instance's mesh tree in object
     // space, where t is the same (geometry.hpp:212-229).
-    if (sc.inst_root != NONE) {
+    if (INST && sc.inst_root != NONE) {
         st.reset();
         uint2 e = make_uint2(sc.inst_root, __float_as_uint(tmin));
         bool have = true;
@@ -260,9 +289,9 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
 template <int S, int BLOCK>
 __device__ __forceinline__ LaneStack<S, BLOCK> lane_stack(uint2* lds, uint2* spill) {
     LaneStack<S, BLOCK> st;
-    st.lds = lds + threadIdx.x;
+    st.lds = (lds_u64*)(lds) + threadIdx.x;
     const uint32_t stride = gridDim.x * BLOCK;
-    st.spill = spill + (blockIdx.x * BLOCK + threadIdx.x);
+    st.spill = (gbl_u64*)(spill) + (blockIdx.x * BLOCK + threadIdx.x);
     st.stride = stride;
     st.top = st.base = 0;
     return st;
