@@ -1,9 +1,10 @@
 // The wavefront path loop: render_row_linear / ray_color (renderer.hpp:100-149,
 // 366-416) split into stages over SoA queues in HBM.
 //
-//   raygen   camera rays for one batch of paths (pixels x samples)
-//   extend   closest hit of every queued ray            (geometry::hit)
-//   shade    emission, scatter, light sample, roulette  (ray_color's body)
+//   extend   closest hit of every queued ray            (geometry::hit);
+//            at bounce 0 the batch's camera rays, made in the kernel
+//   shade    emission, scatter, light sample, roulette  (ray_color's body);
+//            at bounce 0 the path state starts in registers
 //   shadow   any hit of every queued light sample       (geometry::occluded)
 //   ...      extend/shade/shadow again for each bounce, up to max_bounces
 //   accumulate  per pixel, the batch's samples in sample order
@@ -66,53 +67,68 @@ __device__ __forceinline__ float schlick(float cosine, float ratio) {
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_raygen(FrameParams fp) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= fp.P) return;
-    const FrameConst fc = *fp.frame;
-    const uint32_t batch = fc.batch;
+// Path p of the current batch: its pixel, sample number and camera ray
+// (renderer.hpp:379-385, camera.hpp:45-47). Paths are numbered pixel-major
+// within the band, spb samples of a pixel next to each other, so a wave's
+// camera rays are neighbours. Recomputed where needed rather than stored:
+// bounce 0 reads no ray or path state from HBM.
+struct CameraSample {
+    f3 o, d;
+    uint32_t pseed, index;
+    int i, j;
+};
+
+__device__ __forceinline__ CameraSample camera_sample(const FrameParams& fp, const FrameConst& fc, uint32_t p) {
+    CameraSample cs;
     const uint32_t pixel = p / fp.spb, s = p % fp.spb;
-    const int i = static_cast<int>(pixel % fp.W);
+    cs.i = static_cast<int>(pixel % fp.W);
     const int k = static_cast<int>(pixel / fp.W);   // row within the band, top first
-    const int j = fp.j_hi - 1 - k;
-    const uint32_t index = static_cast<uint32_t>(fc.first_sample) + batch * fp.spb + s;
-    const uint32_t pseed = pixel_seed(fc.seed, i, j);
+    cs.j = fp.j_hi - 1 - k;
+    cs.index = static_cast<uint32_t>(fc.first_sample) + fc.batch * fp.spb + s;
+    cs.pseed = pixel_seed(fc.seed, cs.i, cs.j);
     float du, dv;
-    qmc_pair(pseed, index, 0, du, dv);
-    // renderer.hpp:383-385, camera.hpp:45-47
-    const float u = (static_cast<float>(i) + du) / static_cast<float>(fp.W - 1);
-    const float v = (static_cast<float>(j) + dv) / static_cast<float>(fp.H - 1);
-    const f3 origin = xyz(fp.sc.cam[0]);
-    const f3 d = xyz(fp.sc.cam[1]) + xyz(fp.sc.cam[2]) * u + xyz(fp.sc.cam[3]) * v - origin;
-    fp.ro[0][p] = make_float4(origin.x, origin.y, origin.z, T_MIN);
-    fp.rd[0][p] = make_float4(d.x, d.y, d.z, T_MAX);
-    fp.rpath[0][p] = p;
-    fp.pa[p] = make_float4(1, 1, 1, __uint_as_float(0u));
-    fp.pb[p] = make_float4(0, 0, 0, __uint_as_float(0u));
-    const uint64_t key = path_key(fc.seed, i, j, index);
-    fp.pkey[p] = make_uint2(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32));
-    fp.pq[p] = make_uint2(pseed, index);
-    if (p == 0) fp.ctrl[0].n_rays = fp.P;
+    qmc_pair(cs.pseed, cs.index, 0, du, dv);
+    const float u = (static_cast<float>(cs.i) + du) / static_cast<float>(fp.W - 1);
+    const float v = (static_cast<float>(cs.j) + dv) / static_cast<float>(fp.H - 1);
+    cs.o = xyz(fp.sc.cam[0]);
+    cs.d = xyz(fp.sc.cam[1]) + xyz(fp.sc.cam[2]) * u + xyz(fp.sc.cam[3]) * v - cs.o;
+    return cs;
 }
 
-template <int S, bool COUNT, bool INST>
+// PRIMARY: bounce 0, whose rays are the batch's camera rays, made here.
+template <int S, bool COUNT, bool INST, bool PRIMARY>
 __global__ __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_extend(FrameParams fp, int b) {
     __shared__ uint2 lds[S * TRACE_BLOCK];
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
     Visits vis;
     Ctrl* c = fp.ctrl + b;
-    const uint32_t n = c->n_rays;
+    const uint32_t n = PRIMARY ? fp.P : c->n_rays;
+    if (PRIMARY && blockIdx.x == 0 && threadIdx.x == 0) c->n_rays = fp.P;
     const float4* ro = fp.ro[b & 1];
     const float4* rd = fp.rd[b & 1];
+    FrameConst fc;
+    if (PRIMARY) fc = *fp.frame;
     while (true) {
         const uint32_t base = wave_fetch(&c->w_extend, 64);
         if (base >= n) break;
         const uint32_t q = base + lane_id();
         if (q < n) {
-            const float4 a = ro[q], bb = rd[q];
+            f3 o, d;
+            float tmin = T_MIN, tmax = T_MAX;
+            if (PRIMARY) {
+                const CameraSample cs = camera_sample(fp, fc, q);
+                o = cs.o;
+                d = cs.d;
+            } else {
+                const float4 a = ro[q], bb = rd[q];
+                o = xyz(a);
+                d = xyz(bb);
+                tmin = a.w;
+                tmax = bb.w;
+            }
             Hit h;
-            trace_ray<false, COUNT, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st, &vis);
+            trace_ray<false, COUNT, INST>(fp.sc, o, d, tmin, tmax, h, st, &vis);
             fp.hits[q] = h;
         }
     }
@@ -147,10 +163,15 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_sha
 }
 
 // ray_color's loop body for bounce b (renderer.hpp:109-146).
+// PRIMARY: bounce 0; the camera ray and the fresh path state are rebuilt
+// from the path number instead of read back.
+template <bool PRIMARY>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     const SceneView& sc = fp.sc;
     Ctrl* c = fp.ctrl + b;
-    const uint32_t n = c->n_rays;
+    const uint32_t n = PRIMARY ? fp.P : c->n_rays;
+    FrameConst fc;
+    if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1, nxt = cur ^ 1;
     const uint32_t stride = gridDim.x * SHADE_BLOCK;
     for (uint32_t wbase = blockIdx.x * SHADE_BLOCK + (threadIdx.x & ~63u); wbase < n; wbase += stride) {
@@ -162,17 +183,38 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
         bool backface = false;
         uint32_t p = 0;
         if (active) {
-            p = fp.rpath[cur][q];
-            const float4 o4 = fp.ro[cur][q], d4 = fp.rd[cur][q];
             const Hit h = fp.hits[q];
-            const float4 A = fp.pa[p], B = fp.pb[p];
-            const uint2 kk = fp.pkey[p], pq = fp.pq[p];
-            const uint64_t key = (static_cast<uint64_t>(kk.y) << 32) | kk.x;
-            f3 thr = xyz(A), rad = xyz(B);
-            uint32_t cnt = __float_as_uint(A.w);
-            const bool sampled = (__float_as_uint(B.w) & 1u) != 0;
+            f3 o, d, thr, rad;
+            uint32_t cnt;
+            bool sampled;
+            uint64_t key;
+            uint2 pq;
+            if (PRIMARY) {
+                p = q;
+                const CameraSample cs = camera_sample(fp, fc, p);
+                o = cs.o;
+                d = cs.d;
+                thr = mk(1, 1, 1);
+                rad = mk(0, 0, 0);
+                cnt = 0;
+                sampled = false;
+                key = path_key(fc.seed, cs.i, cs.j, cs.index);
+                pq = make_uint2(cs.pseed, cs.index);
+            } else {
+                p = fp.rpath[cur][q];
+                const float4 o4 = fp.ro[cur][q], d4 = fp.rd[cur][q];
+                const float4 A = fp.pa[p], B = fp.pb[p];
+                const uint2 kk = fp.pkey[p];
+                pq = fp.pq[p];
+                key = (static_cast<uint64_t>(kk.y) << 32) | kk.x;
+                thr = xyz(A);
+                rad = xyz(B);
+                cnt = __float_as_uint(A.w);
+                sampled = (__float_as_uint(B.w) & 1u) != 0;
+                o = xyz(o4);
+                d = xyz(d4);
+            }
             uint32_t flags = 0;
-            const f3 o = xyz(o4), d = xyz(d4);
             if (h.kind == 0) {
                 // renderer.hpp:24-29
                 f3 bg = mk(0, 0, 0);
@@ -337,8 +379,16 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 }
                 next_o = pnt;
             }
-            fp.pa[p] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(cnt));
+            // The radiance always (accumulate reads it); the rest only for a
+            // path that goes on.
             fp.pb[p] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(flags));
+            if (cont) {
+                fp.pa[p] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(cnt));
+                if (PRIMARY) {
+                    fp.pkey[p] = make_uint2(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32));
+                    fp.pq[p] = pq;
+                }
+            }
         }
         // Every lane of the wave reaches the appends.
         const uint32_t slot = wave_append(&fp.ctrl[b + 1].n_rays, cont);
@@ -449,13 +499,18 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams fp, float* rgb, flo
 template <int S, bool INST>
 static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev_e,
                         hipEvent_t* ev_s) {
-    k_raygen<<<(fp.P + 255) / 256, 256, 0, stream>>>(fp);
     for (int b = 0; b < fp.depth; ++b) {
         if (ev_e) (void)hipEventRecord(ev_e[2 * b], stream);
-        if (fp.visits) k_extend<S, true, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
-        else k_extend<S, false, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        if (b == 0) {
+            if (fp.visits) k_extend<S, true, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            else k_extend<S, false, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        } else {
+            if (fp.visits) k_extend<S, true, INST, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            else k_extend<S, false, INST, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        }
         if (ev_e) (void)hipEventRecord(ev_e[2 * b + 1], stream);
-        k_shade<<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        if (b == 0) k_shade<true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        else k_shade<false><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
         if (ev_s) (void)hipEventRecord(ev_s[2 * b], stream);
         k_shadow<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
         if (ev_s) (void)hipEventRecord(ev_s[2 * b + 1], stream);
