@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the library's choice)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also copy MIB MiB device to device (a kernel of known traffic, to scale HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -164,6 +166,13 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(ptsd, W, H, spp, args.cpu_seconds)
         log(f"cpu baseline: {json.dumps(cpu)}")
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        a, b = pt.DeviceBuffer(nbytes), pt.DeviceBuffer(nbytes)
+        pt._check(pt.library().pt_device_copy(pt.ctypes.c_void_p(b.ptr), pt.ctypes.c_void_p(a.ptr), nbytes, 2))
+        log(f"copy probe: {nbytes} bytes device to device")
+        a.free()
+        b.free()
     image.free()
     scene.close()
 

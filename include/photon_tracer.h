@@ -187,7 +187,8 @@ pt_status pt_render(pt_scene* scene, const pt_render_params* params, float* rgb,
  * reference, which has no device. */
 pt_status pt_device_alloc(size_t bytes, void** ptr);
 pt_status pt_device_free(void* ptr);
-pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int to_device);
+/* kind: 0 device to host, 1 host to device, 2 device to device. */
+pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int kind);
 
 #ifdef __cplusplus
 }

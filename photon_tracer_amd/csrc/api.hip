@@ -645,9 +645,11 @@ pt_status pt_device_alloc(size_t bytes, void** ptr) {
 
 pt_status pt_device_free(void* ptr) { PT_GUARD(if (ptr) HIPC(hipFree(ptr))); }
 
-pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int to_device) {
+pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int kind) {
     if ((!dst || !src) && bytes) return pt::fail(PT_ERR_INVALID, "pt_device_copy: null pointer");
-    PT_GUARD(if (bytes) HIPC(hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost)));
+    if (kind < 0 || kind > 2) return pt::fail(PT_ERR_INVALID, "pt_device_copy: kind must be 0, 1 or 2");
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    PT_GUARD(if (bytes) HIPC(hipMemcpy(dst, src, bytes, k)));
 }
 
 }  // extern "C"
