@@ -477,7 +477,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         if (paths >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render: batch too large");
         const int S = stack_entries_for(s, P.stack_entries);
         const int trace_grid = trace_grid_for(s, S);
-        const int shade_grid = std::max(1, s->cus * 8);
+        const int shade_grid = std::max(1, s->cus * 2);  // blocks of 1024
         ensure_spill(s, trace_grid);
         const bool want_aux = aux != nullptr;
 

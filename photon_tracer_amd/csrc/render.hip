@@ -21,20 +21,48 @@ namespace ptd {
 
 namespace {
 
-constexpr int SHADE_BLOCK = 256;
+constexpr int SHADE_BLOCK = 1024;
+constexpr int SHADE_WAVES = SHADE_BLOCK / 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
 
-// Slot for this lane in a queue whose length is *counter, for the lanes of the
-// wave where pred holds. Called by every lane of the wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    const uint32_t cnt = static_cast<uint32_t>(__popcll(mask));
-    uint32_t base = 0;
-    if (lane_id() == 0 && cnt) base = atomicAdd(counter, cnt);
-    base = __shfl(base, 0, 64);
-    const unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
-    return base + static_cast<uint32_t>(__popcll(below));
+// Slots in K queues for the lanes of a block where pred[k] holds: each wave
+// counts its lanes with a ballot, the block adds up its waves in LDS, and one
+// thread takes the block's run of each queue with a single atomic (the
+// counters are shared by the whole grid across all XCDs, where atomics on one
+// address are slow). Called by every thread of the block.
+template <int K>
+struct BlockAppend {
+    uint32_t count[K][SHADE_WAVES];
+    uint32_t base[K][SHADE_WAVES];
+};
+
+template <int K>
+__device__ __forceinline__ void block_append(BlockAppend<K>& sh, uint32_t* const* counters, const bool* pred,
+                                             uint32_t* slot) {
+    const uint32_t w = threadIdx.x >> 6;
+    unsigned long long mask[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        mask[k] = __ballot(pred[k]);
+        if (lane_id() == 0) sh.count[k][w] = static_cast<uint32_t>(__popcll(mask[k]));
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        const int k = threadIdx.x;
+        uint32_t total = 0;
+        for (int i = 0; i < SHADE_WAVES; ++i) total += sh.count[k][i];
+        uint32_t at = total ? atomicAdd(counters[k], total) : 0;
+        for (int i = 0; i < SHADE_WAVES; ++i) {
+            sh.base[k][i] = at;
+            at += sh.count[k][i];
+        }
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane_id()) - 1ull;
+#pragma unroll
+    for (int k = 0; k < K; ++k) slot[k] = sh.base[k][w] + static_cast<uint32_t>(__popcll(mask[k] & below));
+    __syncthreads();  // sh is reused by the next iteration
 }
 
 // sample_2d (sampler.hpp:107-124): QMC for pairs up to 2 * qmc_bounces,
@@ -109,11 +137,13 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_ext
     const float4* rd = fp.rd[b & 1];
     FrameConst fc;
     if (PRIMARY) fc = *fp.frame;
+    const uint32_t groups = fetch_groups(n);
     while (true) {
-        const uint32_t base = wave_fetch(&c->w_extend, 64);
+        const uint32_t base = wave_fetch(&c->w_extend, 64 * groups);
         if (base >= n) break;
-        const uint32_t q = base + lane_id();
-        if (q < n) {
+        for (uint32_t g = 0; g < groups; ++g) {
+            const uint32_t q = base + 64 * g + lane_id();
+            if (q >= n) break;
             f3 o, d;
             float tmin = T_MIN, tmax = T_MAX;
             if (PRIMARY) {
@@ -142,11 +172,13 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_sha
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
     Ctrl* c = fp.ctrl + b;
     const uint32_t n = c->n_shadow;
+    const uint32_t groups = fetch_groups(n);
     while (true) {
-        const uint32_t base = wave_fetch(&c->w_shadow, 64);
+        const uint32_t base = wave_fetch(&c->w_shadow, 64 * groups);
         if (base >= n) break;
-        const uint32_t q = base + lane_id();
-        if (q < n) {
+        for (uint32_t g = 0; g < groups; ++g) {
+            const uint32_t q = base + 64 * g + lane_id();
+            if (q >= n) break;
             const float4 a = fp.so[q], bb = fp.sd[q];
             Hit h;
             if (!trace_ray<true, false, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
@@ -173,9 +205,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     FrameConst fc;
     if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1, nxt = cur ^ 1;
+    __shared__ BlockAppend<3> sh;
+    uint32_t* const counters[3] = {&fp.ctrl[b + 1].n_rays, &c->n_shadow, &c->n_backface};
     const uint32_t stride = gridDim.x * SHADE_BLOCK;
-    for (uint32_t wbase = blockIdx.x * SHADE_BLOCK + (threadIdx.x & ~63u); wbase < n; wbase += stride) {
-        const uint32_t q = wbase + lane_id();
+    // The block steps through the queue together (block_append synchronises).
+    for (uint32_t bbase = blockIdx.x * SHADE_BLOCK; bbase < n; bbase += stride) {
+        const uint32_t q = bbase + threadIdx.x;
         const bool active = q < n;
         bool cont = false, shadow = false;
         f3 next_o = mk(0, 0, 0), next_d = mk(0, 0, 0), sh_d = mk(0, 0, 0), sh_l = mk(0, 0, 0);
@@ -390,21 +425,20 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 }
             }
         }
-        // Every lane of the wave reaches the appends.
-        const uint32_t slot = wave_append(&fp.ctrl[b + 1].n_rays, cont);
+        // Every thread of the block reaches the appends.
+        const bool pred[3] = {cont, shadow, backface};
+        uint32_t slot[3];
+        block_append<3>(sh, counters, pred, slot);
         if (cont) {
-            fp.ro[nxt][slot] = make_float4(next_o.x, next_o.y, next_o.z, T_MIN);
-            fp.rd[nxt][slot] = make_float4(next_d.x, next_d.y, next_d.z, T_MAX);
-            fp.rpath[nxt][slot] = p;
+            fp.ro[nxt][slot[0]] = make_float4(next_o.x, next_o.y, next_o.z, T_MIN);
+            fp.rd[nxt][slot[0]] = make_float4(next_d.x, next_d.y, next_d.z, T_MAX);
+            fp.rpath[nxt][slot[0]] = p;
         }
-        const uint32_t sslot = wave_append(&c->n_shadow, shadow);
         if (shadow) {
-            fp.so[sslot] = make_float4(next_o.x, next_o.y, next_o.z, 0.001f);
-            fp.sd[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_tmax);
-            fp.sl[sslot] = make_float4(sh_l.x, sh_l.y, sh_l.z, __uint_as_float(p));
+            fp.so[slot[1]] = make_float4(next_o.x, next_o.y, next_o.z, 0.001f);
+            fp.sd[slot[1]] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_tmax);
+            fp.sl[slot[1]] = make_float4(sh_l.x, sh_l.y, sh_l.z, __uint_as_float(p));
         }
-        const unsigned long long bf = __ballot(backface);
-        if (lane_id() == 0 && bf) atomicAdd(&c->n_backface, static_cast<uint32_t>(__popcll(bf)));
     }
 }
 

@@ -14,11 +14,13 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_tra
     auto st = lane_stack<S, TRACE_BLOCK>(lds, spill);
     const uint32_t lane = threadIdx.x & 63;
     Visits vis;
+    const uint32_t groups = fetch_groups(n);
     while (true) {
-        const uint32_t base = wave_fetch(work, 64);
+        const uint32_t base = wave_fetch(work, 64 * groups);
         if (base >= n) break;
-        const uint32_t q = base + lane;
-        if (q < n) {
+        for (uint32_t g = 0; g < groups; ++g) {
+            const uint32_t q = base + 64 * g + lane;
+            if (q >= n) break;
             const float4 a = rays[2 * q], b = rays[2 * q + 1];
             Hit h;
             if (ANY) {

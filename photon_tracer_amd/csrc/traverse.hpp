@@ -316,4 +316,15 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t* counter, uint32_t n) {
     return __shfl(base, 0, 64);
 }
 
+// Groups of 64 items a wave takes per fetch from a queue of n items. One
+// counter is shared by every wave of the grid across all XCDs, where each
+// atomic on it is slow; large queues are handed out up to 512 items at a
+// time, sized so every wave still makes about 16 fetches (a finer split of
+// the last pieces, which decide when the kernel ends).
+__device__ __forceinline__ uint32_t fetch_groups(uint32_t n) {
+    const uint32_t waves = gridDim.x * (blockDim.x / 64);
+    const uint32_t g = n / (waves * 64u * 16u);
+    return g < 1 ? 1 : g > 8 ? 8 : g;
+}
+
 }  // namespace ptd
