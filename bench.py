@@ -147,14 +147,14 @@ def main():
         # stage's visits, one timing each of its launches with HIP events.
         _, _, cst = scene.render(W, H, spp, seed=1, count=True, stack_entries=args.stack)
         _, _, tst = scene.render(W, H, spp, seed=1, timing=True, stack_entries=args.stack)
-        queries = cst.rays - cst.shadow_rays
+        queries = cst.rays  # closest-hit and shadow queries, both traced by the stage
         algo = pt.QUERY_BYTES * queries + sum(v * b for v, b in zip(cst.visits, pt.VISIT_BYTES))
         launches = 10 * tst.batches
         achieved = algo / (tst.extend_ms * 1e-3) / 1e9
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_extend (closest-hit traversal stage)",
+            "kernel": "k_extend (traversal stage: closest-hit rays + the previous bounce's shadow rays)",
             "launches": launches, "avg_launch_ms": round(tst.extend_ms / launches, 4),
             "algorithmic_bytes_per_launch": int(algo / launches),
             "per_query": {"queries": queries, "nodes": cst.visits[0] / queries, "leafrefs": cst.visits[1] / queries,

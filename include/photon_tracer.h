@@ -158,7 +158,7 @@ enum {
     PT_RENDER_NO_NEE = 1,        /* world.sample_lights = false (main.cpp --no-nee) */
     PT_RENDER_NO_GRAPH = 2,      /* launch the bounce loop eagerly, not as a hipGraph */
     PT_RENDER_TIMING = 4,        /* eager, with HIP events around every traversal launch */
-    PT_RENDER_COUNT = 8,         /* eager, closest-hit stage counts its visits (stats->visits) */
+    PT_RENDER_COUNT = 8,         /* eager, the traversal stages count their visits (stats->visits) */
     PT_RENDER_DEVICE_OUTPUT = 16 /* rgb and aux are device addresses (pt_device_alloc): no copy to the host */
 };
 
@@ -168,10 +168,11 @@ typedef struct {
     uint64_t primary_rays;
     uint64_t shadow_rays;
     double render_ms;          /* device time of the render, HIP events */
-    double extend_ms;          /* with PT_RENDER_TIMING: time in closest-hit traversal */
-    double shadow_ms;          /* with PT_RENDER_TIMING: time in any-hit traversal */
-    uint64_t visits[6];        /* with PT_RENDER_COUNT: closest-hit node, leaf-ref,
-                                  triangle, sphere, instance, large-sphere visits */
+    double extend_ms;          /* with PT_RENDER_TIMING: time in the traversal stages
+                                  (closest-hit rays and the previous bounce's shadow rays) */
+    double shadow_ms;          /* reserved, 0 (shadow rays are traced in the traversal stage) */
+    uint64_t visits[6];        /* with PT_RENDER_COUNT: node, leaf-ref, triangle, sphere,
+                                  instance, large-sphere visits of all traversal queries */
     int32_t batches;           /* batches the samples were split into */
     int32_t samples_per_batch; /* samples per pixel in each batch */
 } pt_render_stats;

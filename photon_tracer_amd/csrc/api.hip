@@ -560,23 +560,21 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         const bool timing = stats && (P.flags & PT_RENDER_TIMING);
         if (timing) {
             // Eager, with an event pair around every traversal launch.
-            std::vector<hipEvent_t> ev(4 * P.max_bounces);
+            std::vector<hipEvent_t> ev(2 * P.max_bounces);
             for (auto& e : ev) HIPC(hipEventCreate(&e));
             for (int b = 0; b < batches; ++b) {
-                HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, ev.data(), ev.data() + 2 * P.max_bounces));
+                HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, ev.data()));
                 HIPC(hipStreamSynchronize(s->stream));
                 for (int k = 0; k < P.max_bounces; ++k) {
                     float ms = 0;
                     HIPC(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
                     extend_ms += ms;
-                    HIPC(hipEventElapsedTime(&ms, ev[2 * P.max_bounces + 2 * k], ev[2 * P.max_bounces + 2 * k + 1]));
-                    shadow_ms += ms;
                 }
             }
             for (auto& e : ev) (void)hipEventDestroy(e);
         } else if ((P.flags & PT_RENDER_NO_GRAPH) || counting) {
             for (int b = 0; b < batches; ++b)
-                HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr, nullptr));
+                HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr));
         } else {
             // The batch's whole bounce loop as one graph, replayed per batch
             // and kept for the next render of the same shape; kernels read
@@ -588,7 +586,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
                 s->drop_graph();
                 hipGraph_t graph;
                 HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                const hipError_t le = ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr, nullptr);
+                const hipError_t le = ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr);
                 const hipError_t ce = hipStreamEndCapture(s->stream, &graph);
                 HIPC(le);
                 HIPC(ce);

@@ -15,8 +15,8 @@ struct Ctrl {
     uint32_t n_rays;      // closest-hit queue size of this bounce
     uint32_t n_shadow;    // shadow queue size of this bounce
     uint32_t n_backface;  // light samples that count as rays but need no trace
-    uint32_t w_extend;    // next item of the closest-hit queue to hand out
-    uint32_t w_shadow;    // next item of the shadow queue to hand out
+    uint32_t w_extend;    // next item of the traversal stage (shadow + closest-hit queues) to hand out
+    uint32_t w_unused;
     uint32_t pad[3];
 };
 constexpr int MAX_BOUNCES = 64;
@@ -57,7 +57,7 @@ struct FrameParams {
     FrameConst* frame;        // seed, first sample, batch number
     unsigned long long* totals;  // [0] closest-hit rays, [1] shadow rays, [2] primary rays
     uint2* spill;             // traversal stack spill area
-    unsigned long long* visits;  // null, or six totals the closest-hit stage counts into
+    unsigned long long* visits;  // null, or six totals the traversal stage counts into
 };
 
 // Traversal kernels for ray batches (pt_trace).
@@ -68,9 +68,10 @@ hipError_t launch_trace(const SceneView& sc, const float4* rays, uint32_t n, Hit
 int trace_blocks_per_cu(int stack_entries);
 constexpr int TRACE_BLOCK = 256;
 
-// The bounce loop of one batch of paths.
+// The bounce loop of one batch of paths. ev: null, or 2 * depth events
+// recorded around each bounce's traversal launch.
 hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
-                        hipEvent_t* ev_extend, hipEvent_t* ev_shadow);
+                        hipEvent_t* ev);
 hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStream_t stream);
 hipError_t launch_clear_frame(const FrameParams& fp, hipStream_t stream);
 

@@ -123,7 +123,11 @@ __device__ __forceinline__ CameraSample camera_sample(const FrameParams& fp, con
     return cs;
 }
 
-// PRIMARY: bounce 0, whose rays are the batch's camera rays, made here.
+// The traversal stage of bounce b: the light samples queued by bounce b - 1
+// (any hit, geometry::occluded: an unoccluded sample adds its radiance to its
+// path) and the rays of bounce b (closest hit, geometry::hit), drained by one
+// persistent grid from one counter, shadow rays first. PRIMARY: bounce 0,
+// whose rays are the batch's camera rays, made here; it has no shadow rays.
 template <int S, bool COUNT, bool INST, bool PRIMARY>
 __global__ __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_extend(FrameParams fp, int b) {
@@ -131,7 +135,9 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_ext
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
     Visits vis;
     Ctrl* c = fp.ctrl + b;
-    const uint32_t n = PRIMARY ? fp.P : c->n_rays;
+    const uint32_t n_shadow = PRIMARY ? 0u : fp.ctrl[b - 1].n_shadow;
+    const uint32_t n_rays = PRIMARY ? fp.P : c->n_rays;
+    const uint32_t n = n_shadow + n_rays;
     if (PRIMARY && blockIdx.x == 0 && threadIdx.x == 0) c->n_rays = fp.P;
     const float4* ro = fp.ro[b & 1];
     const float4* rd = fp.rd[b & 1];
@@ -142,8 +148,23 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_ext
         const uint32_t base = wave_fetch(&c->w_extend, 64 * groups);
         if (base >= n) break;
         for (uint32_t g = 0; g < groups; ++g) {
-            const uint32_t q = base + 64 * g + lane_id();
-            if (q >= n) break;
+            const uint32_t item = base + 64 * g + lane_id();
+            if (item >= n) break;
+            Hit h;
+            if (!PRIMARY && item < n_shadow) {
+                const float4 a = fp.so[item], bb = fp.sd[item];
+                if (!trace_ray<true, COUNT, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st, &vis)) {
+                    const float4 l = fp.sl[item];
+                    const uint32_t p = __float_as_uint(l.w);
+                    float4 r = fp.pb[p];
+                    r.x = r.x + l.x;
+                    r.y = r.y + l.y;
+                    r.z = r.z + l.z;
+                    fp.pb[p] = r;
+                }
+                continue;
+            }
+            const uint32_t q = item - n_shadow;
             f3 o, d;
             float tmin = T_MIN, tmax = T_MAX;
             if (PRIMARY) {
@@ -157,41 +178,11 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_ext
                 tmin = a.w;
                 tmax = bb.w;
             }
-            Hit h;
             trace_ray<false, COUNT, INST>(fp.sc, o, d, tmin, tmax, h, st, &vis);
             fp.hits[q] = h;
         }
     }
     if (COUNT) add_visits(fp.visits, vis);
-}
-
-template <int S, bool INST>
-__global__ __launch_bounds__(TRACE_BLOCK)
-__attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_shadow(FrameParams fp, int b) {
-    __shared__ uint2 lds[S * TRACE_BLOCK];
-    auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
-    Ctrl* c = fp.ctrl + b;
-    const uint32_t n = c->n_shadow;
-    const uint32_t groups = fetch_groups(n);
-    while (true) {
-        const uint32_t base = wave_fetch(&c->w_shadow, 64 * groups);
-        if (base >= n) break;
-        for (uint32_t g = 0; g < groups; ++g) {
-            const uint32_t q = base + 64 * g + lane_id();
-            if (q >= n) break;
-            const float4 a = fp.so[q], bb = fp.sd[q];
-            Hit h;
-            if (!trace_ray<true, false, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
-                const float4 l = fp.sl[q];
-                const uint32_t p = __float_as_uint(l.w);
-                float4 r = fp.pb[p];
-                r.x = r.x + l.x;
-                r.y = r.y + l.y;
-                r.z = r.z + l.z;
-                fp.pb[p] = r;
-            }
-        }
-    }
 }
 
 // ray_color's loop body for bounce b (renderer.hpp:109-146).
@@ -531,10 +522,9 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams fp, float* rgb, flo
 }
 
 template <int S, bool INST>
-static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev_e,
-                        hipEvent_t* ev_s) {
+static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev) {
     for (int b = 0; b < fp.depth; ++b) {
-        if (ev_e) (void)hipEventRecord(ev_e[2 * b], stream);
+        if (ev) (void)hipEventRecord(ev[2 * b], stream);
         if (b == 0) {
             if (fp.visits) k_extend<S, true, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
             else k_extend<S, false, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
@@ -542,33 +532,32 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
             if (fp.visits) k_extend<S, true, INST, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
             else k_extend<S, false, INST, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
         }
-        if (ev_e) (void)hipEventRecord(ev_e[2 * b + 1], stream);
+        if (ev) (void)hipEventRecord(ev[2 * b + 1], stream);
         if (b == 0) k_shade<true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
         else k_shade<false><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
-        if (ev_s) (void)hipEventRecord(ev_s[2 * b], stream);
-        k_shadow<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
-        if (ev_s) (void)hipEventRecord(ev_s[2 * b + 1], stream);
     }
+    // No light samples are taken at the last bounce (renderer.hpp:127-130),
+    // so no shadow rays are left over.
     k_accumulate<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp);
     k_finish<<<1, 64, 0, stream>>>(fp);
 }
 
 hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
-                        hipEvent_t* ev_extend, hipEvent_t* ev_shadow) {
+                        hipEvent_t* ev) {
     if (fp.depth < 1 || fp.depth >= MAX_BOUNCES) return hipErrorInvalidValue;
     const bool inst = fp.sc.inst_root != NONE;
     switch (stack_entries) {
         case 8:
-            if (inst) launch_loop<8, true>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
-            else launch_loop<8, false>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            if (inst) launch_loop<8, true>(fp, trace_grid, shade_grid, stream, ev);
+            else launch_loop<8, false>(fp, trace_grid, shade_grid, stream, ev);
             break;
         case 16:
-            if (inst) launch_loop<16, true>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
-            else launch_loop<16, false>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            if (inst) launch_loop<16, true>(fp, trace_grid, shade_grid, stream, ev);
+            else launch_loop<16, false>(fp, trace_grid, shade_grid, stream, ev);
             break;
         case 32:
-            if (inst) launch_loop<32, true>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
-            else launch_loop<32, false>(fp, trace_grid, shade_grid, stream, ev_extend, ev_shadow);
+            if (inst) launch_loop<32, true>(fp, trace_grid, shade_grid, stream, ev);
+            else launch_loop<32, false>(fp, trace_grid, shade_grid, stream, ev);
             break;
         default: return hipErrorInvalidValue;
     }
