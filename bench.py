@@ -147,7 +147,7 @@ def main():
         # stage's visits, one timing each of its launches with HIP events.
         _, _, cst = scene.render(W, H, spp, seed=1, count=True, stack_entries=args.stack)
         _, _, tst = scene.render(W, H, spp, seed=1, timing=True, stack_entries=args.stack)
-        queries = cst.rays  # closest-hit and shadow queries, both traced by the stage
+        queries = cst.visits[6]  # closest-hit and shadow queries traced by the traversal-stage launches
         algo = pt.QUERY_BYTES * queries + sum(v * b for v, b in zip(cst.visits, pt.VISIT_BYTES))
         launches = 10 * tst.batches
         achieved = algo / (tst.extend_ms * 1e-3) / 1e9

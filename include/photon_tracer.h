@@ -159,7 +159,8 @@ enum {
     PT_RENDER_NO_GRAPH = 2,      /* launch the bounce loop eagerly, not as a hipGraph */
     PT_RENDER_TIMING = 4,        /* eager, with HIP events around every traversal launch */
     PT_RENDER_COUNT = 8,         /* eager, the traversal stages count their visits (stats->visits) */
-    PT_RENDER_DEVICE_OUTPUT = 16 /* rgb and aux are device addresses (pt_device_alloc): no copy to the host */
+    PT_RENDER_DEVICE_OUTPUT = 16,/* rgb and aux are device addresses (pt_device_alloc): no copy to the host */
+    PT_RENDER_NO_TAIL = 32       /* every bounce stage by stage: no tail launch for the last, small queues */
 };
 
 /* Render statistics. */
@@ -171,8 +172,9 @@ typedef struct {
     double extend_ms;          /* with PT_RENDER_TIMING: time in the traversal stages
                                   (closest-hit rays and the previous bounce's shadow rays) */
     double shadow_ms;          /* reserved, 0 (shadow rays are traced in the traversal stage) */
-    uint64_t visits[6];        /* with PT_RENDER_COUNT: node, leaf-ref, triangle, sphere,
-                                  instance, large-sphere visits of all traversal queries */
+    uint64_t visits[7];        /* with PT_RENDER_COUNT, over the traversal-stage launches: node,
+                                  leaf-ref, triangle, sphere, instance and large-sphere visits,
+                                  and the number of queries those launches traced */
     int32_t batches;           /* batches the samples were split into */
     int32_t samples_per_batch; /* samples per pixel in each batch */
 } pt_render_stats;

@@ -34,6 +34,7 @@ RENDER_NO_GRAPH = 2
 RENDER_TIMING = 4
 RENDER_COUNT = 8
 RENDER_DEVICE_OUTPUT = 16
+RENDER_NO_TAIL = 32
 
 # Algorithmic bytes of one traversal query (the ray in, the hit out, and every
 # record the reference's walk must read): 32 + 16 per query, 128 per node slab
@@ -64,7 +65,7 @@ class _RenderStats(ctypes.Structure):
     _fields_ = [
         ("rays", ctypes.c_uint64), ("primary_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
         ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
-        ("visits", ctypes.c_uint64 * 6), ("batches", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
+        ("visits", ctypes.c_uint64 * 7), ("batches", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
     ]
 
 
@@ -195,7 +196,7 @@ class RenderStats:
     render_ms: float
     extend_ms: float
     shadow_ms: float
-    visits: tuple = (0, 0, 0, 0, 0, 0)
+    visits: tuple = (0, 0, 0, 0, 0, 0, 0)
     batches: int = 0
     samples_per_batch: int = 0
 
@@ -265,14 +266,15 @@ class Scene:
     def render(self, width: int, height: int, spp: int, max_bounces: int = 10, seed: int = 0,
                first_sample: int = 0, rows: tuple[int, int] | None = None, nee: bool = True, aux: bool = False,
                graph: bool = True, timing: bool = False, count: bool = False, samples_per_batch: int = 0,
-               stack_entries: int = 0, device_out: "DeviceBuffer | None" = None):
+               stack_entries: int = 0, device_out: "DeviceBuffer | None" = None, tail: bool = True):
         """Renders rows [j0, j1) (from the bottom; default all). Returns (rgb, aux, stats) with rgb of shape
         (rows, width, 3), the band's top row first, as render_row_linear writes rows."""
         j0, j1 = rows if rows else (0, height)
         p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, j0, j1,
                           (0 if nee else RENDER_NO_NEE) | (0 if graph else RENDER_NO_GRAPH) |
                           (RENDER_TIMING if timing else 0) | (RENDER_COUNT if count else 0) |
-                          (RENDER_DEVICE_OUTPUT if device_out is not None else 0), samples_per_batch,
+                          (RENDER_DEVICE_OUTPUT if device_out is not None else 0) |
+                          (0 if tail else RENDER_NO_TAIL), samples_per_batch,
                           stack_entries)
         nrows = j1 - j0
         st = _RenderStats()

@@ -415,7 +415,7 @@ pt_status pt_trace_device(pt_scene* s, const void* d_rays, size_t n, int any_hit
         unsigned long long* counts = nullptr;
         if (visits) {
             s->counts.ensure(64);
-            HIPC(hipMemsetAsync(s->counts.p, 0, 48, s->stream));
+            HIPC(hipMemsetAsync(s->counts.p, 0, 64, s->stream));
             counts = s->counts.as<unsigned long long>();
         }
         HIPC(ptd::launch_trace(s->view, static_cast<const float4*>(d_rays), static_cast<uint32_t>(n),
@@ -541,10 +541,13 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         fp.frame = B.frame.as<ptd::FrameConst>();
         fp.totals = B.totals.as<unsigned long long>();
         fp.spill = s->spill.as<uint2>();
+        // Below four rays per resident traversal lane, a queue no longer fills
+        // the GPU stage by stage and the tail launch takes the paths.
+        fp.tail_threshold = (P.flags & PT_RENDER_NO_TAIL) ? 0u : static_cast<uint32_t>(trace_grid) * ptd::TRACE_BLOCK * 4u;
         const bool counting = stats && (P.flags & PT_RENDER_COUNT);
         if (counting) {
             s->counts.ensure(64);
-            HIPC(hipMemsetAsync(s->counts.p, 0, 48, s->stream));
+            HIPC(hipMemsetAsync(s->counts.p, 0, 64, s->stream));
             fp.visits = s->counts.as<unsigned long long>();
         }
 
@@ -552,7 +555,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         hipEvent_t e0, e1;
         HIPC(hipEventCreate(&e0));
         HIPC(hipEventCreate(&e1));
-        const ptd::FrameConst fc{P.seed, P.first_sample, 0u};
+        const ptd::FrameConst fc{P.seed, P.first_sample, 0u, ~0u, 0u};
         HIPC(hipMemcpyAsync(B.frame.p, &fc, sizeof fc, hipMemcpyHostToDevice, s->stream));
         HIPC(ptd::launch_clear_frame(fp, s->stream));
         HIPC(hipEventRecord(e0, s->stream));
@@ -626,9 +629,9 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             stats->render_ms = render_ms;
             stats->extend_ms = extend_ms;
             stats->shadow_ms = shadow_ms;
-            unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
-            if (counting) HIPC(hipMemcpy(v, s->counts.p, 48, hipMemcpyDeviceToHost));
-            for (int k = 0; k < 6; ++k) stats->visits[k] = v[k];
+            unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
+            if (counting) HIPC(hipMemcpy(v, s->counts.p, 56, hipMemcpyDeviceToHost));
+            for (int k = 0; k < 7; ++k) stats->visits[k] = v[k];
             stats->batches = batches;
             stats->samples_per_batch = spb;
         }

@@ -27,6 +27,8 @@ struct FrameConst {
     uint64_t seed;
     int32_t first_sample;
     uint32_t batch;       // batch number, advanced by k_finish
+    uint32_t tail_from;   // bounce at which the tail launch took over, or ~0u
+    uint32_t pad;
 };
 
 struct FrameParams {
@@ -53,11 +55,12 @@ struct FrameParams {
     float4* paux;             // 2 per path: (first albedo, 0), (first normal, 0)
     float4* accum;            // per pixel: rgb sum
     float4* aux_accum;        // 2 per pixel: (albedo sum, lum^2 sum), (normal sum, 0)
-    Ctrl* ctrl;               // MAX_BOUNCES + 1 entries
+    Ctrl* ctrl;               // MAX_BOUNCES + 1 entries; ctrl[MAX_BOUNCES] holds the tail launch's counts
     FrameConst* frame;        // seed, first sample, batch number
     unsigned long long* totals;  // [0] closest-hit rays, [1] shadow rays, [2] primary rays
     uint2* spill;             // traversal stack spill area
-    unsigned long long* visits;  // null, or six totals the traversal stage counts into
+    unsigned long long* visits;  // null, or seven totals the traversal stage counts into (Visits)
+    uint32_t tail_threshold;  // queues below this many rays go to the tail launch (0: never)
 };
 
 // Traversal kernels for ray batches (pt_trace).

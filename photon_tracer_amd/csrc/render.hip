@@ -131,6 +131,7 @@ __device__ __forceinline__ CameraSample camera_sample(const FrameParams& fp, con
 template <int S, bool COUNT, bool INST, bool PRIMARY>
 __global__ __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_extend(FrameParams fp, int b) {
+    if (!PRIMARY && fp.frame->tail_from <= static_cast<uint32_t>(b)) return;  // the tail launch has these rays
     __shared__ uint2 lds[S * TRACE_BLOCK];
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
     Visits vis;
@@ -185,12 +186,204 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_ext
     if (COUNT) add_visits(fp.visits, vis);
 }
 
-// ray_color's loop body for bounce b (renderer.hpp:109-146).
-// PRIMARY: bounce 0; the camera ray and the fresh path state are rebuilt
-// from the path number instead of read back.
+// The state a path carries from bounce to bounce.
+struct PathState {
+    f3 thr, rad;        // throughput, radiance so far
+    uint32_t cnt;       // draws taken from the path's random stream
+    bool sampled;       // this bounce took a light sample (renderer.hpp:106-108,130)
+    uint64_t key;       // the path's random stream
+    uint2 pq;           // QMC pixel seed and sample index
+};
+
+// What a bounce produced: the next ray (cont: from pnt along dir) and a light
+// sample to test (shadow: from pnt along sh_d up to sh_tmax, adding sh_l if
+// unoccluded); backface: a light sample that counts as a ray but is zero
+// without a test (renderer.hpp:72,76-77).
+struct ShadeResult {
+    bool cont, shadow, backface;
+    f3 pnt, dir, sh_d, sh_l;
+    float sh_tmax;
+};
+
+// ray_color's loop body for bounce b (renderer.hpp:109-146): the ray from o
+// along d, whose closest hit is h, of path p.
+__device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, uint32_t p, const Hit& h, f3 o, f3 d,
+                                                 PathState& ps) {
+    const SceneView& sc = fp.sc;
+    ShadeResult r;
+    r.cont = r.shadow = r.backface = false;
+    r.pnt = r.dir = r.sh_d = r.sh_l = mk(0, 0, 0);
+    r.sh_tmax = 0;
+    const bool was_sampled = ps.sampled;
+    ps.sampled = false;
+    if (h.kind == 0) {
+        // renderer.hpp:24-29
+        f3 bg = mk(0, 0, 0);
+        if (sc.flags & 1u) {
+            const f3 unit = normalize(d);
+            const float t = 0.5f * (unit.y + 1);
+            bg = mk(1, 1, 1) * (1 - t) + mk(0.5f, 0.7f, 1.0f) * t;
+        }
+        ps.rad = ps.rad + mul(ps.thr, bg);
+        if (fp.want_aux && b == 0) {
+            fp.paux[2 * p] = make_float4(1, 1, 1, 0);
+            fp.paux[2 * p + 1] = make_float4(0, 0, 0, 0);
+        }
+        return r;
+    }
+    // The hit record (geometry.hpp:135-151, sphere.hpp:50-58,
+    // triangle.hpp:63-74, geometry.hpp:231-244).
+    const f3 pnt = o + d * h.t;
+    r.pnt = pnt;
+    f3 nrm;
+    bool front;
+    uint32_t mat;
+    bool sampled_light = false;
+    if (h.kind <= 2) {
+        const float4 cr = sc.spheres[2 * h.prim];
+        mat = __float_as_uint(sc.spheres[2 * h.prim + 1].x);
+        const f3 outward = (pnt - xyz(cr)) * (1 / cr.w);
+        front = dot(d, outward) < 0;
+        nrm = front ? outward : -outward;
+        if (h.kind == 2 && fp.nee) {
+            const float4 em = sc.mats[3 * mat + 1];
+            sampled_light = em.x > 0 || em.y > 0 || em.z > 0;
+        }
+    } else {
+        const float4* tr = sc.tris + 7 * h.prim;
+        mat = __float_as_uint(tr[0].w);
+        const f3 n0 = xyz(tr[3]), n1 = xyz(tr[4]), n2 = xyz(tr[5]), fn = xyz(tr[6]);
+        const float b0 = 1 - h.u - h.v;
+        const f3 sn = normalize(n0 * b0 + n1 * h.u + n2 * h.v);
+        if (h.inst == NONE) {
+            front = dot(d, fn) < 0;
+            nrm = front ? sn : -sn;
+        } else {
+            const float4* in = sc.inst + 7 * h.inst;
+            const float4 r0 = in[3], r1 = in[4], r2 = in[5];
+            const f3 ld = mk(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+                             r2.x * d.x + r2.y * d.y + r2.z * d.z);
+            front = dot(ld, fn) < 0;
+            const f3 ln = front ? sn : -sn;
+            // the transpose of to_object (transform.hpp:23-27)
+            nrm = normalize(mk(r0.x * ln.x + r1.x * ln.y + r2.x * ln.z, r0.y * ln.x + r1.y * ln.y + r2.y * ln.z,
+                               r0.z * ln.x + r1.z * ln.y + r2.z * ln.z));
+            const uint32_t im = __float_as_uint(in[6].y);
+            if (im != NONE) mat = im;
+        }
+    }
+    const float4 m0 = sc.mats[3 * mat], m1 = sc.mats[3 * mat + 1], m2 = sc.mats[3 * mat + 2];
+    if (front && !(was_sampled && sampled_light)) ps.rad = ps.rad + mul(ps.thr, xyz(m1));
+
+    // material::scatter (material.hpp:59-137)
+    const int type = static_cast<int>(m0.w);
+    f3 att = xyz(m0), dir = mk(0, 0, 0);
+    bool ok = true;
+    if (type == 0) {
+        float a, bq;
+        sample_2d(ps.pq.x, ps.pq.y, 1 + 2 * b, ps.key, ps.cnt, a, bq);
+        const float rr = sqrtf(a), phi = 2 * PI_F * bq;
+        const f3 nn = nrm;
+        const float sign = copysignf(1.0f, nn.z);
+        const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
+        const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
+        const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
+        dir = t1 * (rr * cosf(phi)) + t2 * (rr * sinf(phi)) + nn * sqrtf(fmaxf(0.0f, 1 - a));
+    } else if (type == 1) {
+        const f3 reflected = reflect(normalize(d), nrm);
+        f3 ball = mk(0, 0, 0);
+        for (int tries = 0; tries < 64; ++tries) {  // vec3.hpp:121-127
+            const float x = -1 + 2 * rand01(ps.key, ps.cnt);
+            const float y = -1 + 2 * rand01(ps.key, ps.cnt);
+            const float z = -1 + 2 * rand01(ps.key, ps.cnt);
+            ball = mk(x, y, z);
+            if (dot(ball, ball) < 1) break;
+        }
+        dir = reflected + ball * m1.w;
+        ok = dot(dir, nrm) > 0;
+    } else if (type == 2) {
+        att = mk(1, 1, 1);
+        const float ior = m2.x;
+        const float ratio = front ? (1 / ior) : ior;
+        const f3 ud = normalize(d);
+        const float cos_theta = fminf(dot(-ud, nrm), 1.0f);
+        const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
+        const bool cannot_refract = ratio * sin_theta > 1;
+        const bool refl = cannot_refract || schlick(cos_theta, ratio) > rand01(ps.key, ps.cnt);
+        dir = refl ? reflect(ud, nrm) : refract(ud, nrm, ratio);
+    } else {
+        ok = false;
+    }
+    if (fp.want_aux && b == 0) {
+        const f3 fa = ok && type != 2 ? att : mk(1, 1, 1);
+        fp.paux[2 * p] = make_float4(fa.x, fa.y, fa.z, 0);
+        fp.paux[2 * p + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0);
+    }
+    if (!ok) return r;
+    const bool nee = type == 0 && fp.nee && sc.n_lights > 0 && b + 1 < fp.depth;
+    if (nee) {
+        // direct_light (renderer.hpp:41-80)
+        const uint32_t nl = sc.n_lights;
+        const uint32_t pick = min(static_cast<uint32_t>(rand01(ps.key, ps.cnt) * nl), nl - 1);
+        const uint32_t ls = sc.lights[pick];
+        const float4 lcr = sc.spheres[2 * ls];
+        const uint32_t lmat = __float_as_uint(sc.spheres[2 * ls + 1].x);
+        const f3 to_center = xyz(lcr) - pnt;
+        const float dist2 = dot(to_center, to_center), r2 = lcr.w * lcr.w;
+        if (dist2 > r2) {
+            const float sin2_max = r2 / dist2;
+            const float cos_max = sqrtf(1 - sin2_max);
+            const float omc = sin2_max / (1 + cos_max);
+            const f3 w = to_center * (1 / sqrtf(dist2));
+            const f3 ax = fabsf(w.x) > 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
+            const f3 uu = normalize(cross(w, ax)), vv = cross(w, uu);
+            float s1, s2;
+            sample_2d(ps.pq.x, ps.pq.y, 2 + 2 * b, ps.key, ps.cnt, s1, s2);
+            const float cos_t = 1 - s1 * omc;
+            const float sin_t = sqrtf(fmaxf(0.0f, 1 - cos_t * cos_t));
+            const float phi = 2 * PI_F * s2;
+            const f3 ldir = uu * (cosf(phi) * sin_t) + vv * (sinf(phi) * sin_t) + w * cos_t;
+            const float cos_surface = dot(ldir, nrm);
+            float t_light = __builtin_huge_valf();
+            if (cos_surface > 0 && sphere_hit_f(lcr, pnt, ldir, 0.001f, t_light)) {
+                const f3 lp = pnt + ldir * t_light;
+                const f3 outward = (lp - xyz(lcr)) * (1 / lcr.w);
+                if (dot(ldir, outward) < 0) {
+                    const float pdf = 1 / (2 * PI_F * omc);
+                    const f3 contrib =
+                        xyz(sc.mats[3 * lmat + 1]) * (cos_surface / PI_F / pdf * static_cast<float>(nl));
+                    r.shadow = true;
+                    r.sh_d = ldir;
+                    r.sh_tmax = t_light * 0.9999f;
+                    r.sh_l = mul(mul(ps.thr, att), contrib);
+                } else {
+                    r.backface = true;
+                }
+            }
+        }
+    }
+    ps.sampled = nee;
+    ps.thr = mul(ps.thr, att);
+    if (b + 1 >= 3) {  // roulette_after (renderer.hpp:22,140-146)
+        const float pm = fmaxf(ps.thr.x, fmaxf(ps.thr.y, ps.thr.z));
+        if (pm < 1) {
+            if (rand01(ps.key, ps.cnt) >= pm) return r;
+            ps.thr = ps.thr * (1 / pm);
+        }
+    }
+    if (b + 1 < fp.depth) {
+        r.cont = true;
+        r.dir = dir;
+    }
+    return r;
+}
+
+// The shade stage of bounce b over its queue. PRIMARY: bounce 0; the camera
+// ray and the fresh path state are rebuilt from the path number instead of
+// read back.
 template <bool PRIMARY>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
-    const SceneView& sc = fp.sc;
+    if (fp.frame->tail_from <= static_cast<uint32_t>(b)) return;  // the tail launch has taken these paths
     Ctrl* c = fp.ctrl + b;
     const uint32_t n = PRIMARY ? fp.P : c->n_rays;
     FrameConst fc;
@@ -202,234 +395,174 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     // The block steps through the queue together (block_append synchronises).
     for (uint32_t bbase = blockIdx.x * SHADE_BLOCK; bbase < n; bbase += stride) {
         const uint32_t q = bbase + threadIdx.x;
-        const bool active = q < n;
-        bool cont = false, shadow = false;
-        f3 next_o = mk(0, 0, 0), next_d = mk(0, 0, 0), sh_d = mk(0, 0, 0), sh_l = mk(0, 0, 0);
-        float sh_tmax = 0;
-        bool backface = false;
+        ShadeResult r;
+        r.cont = r.shadow = r.backface = false;
         uint32_t p = 0;
-        if (active) {
+        if (q < n) {
             const Hit h = fp.hits[q];
-            f3 o, d, thr, rad;
-            uint32_t cnt;
-            bool sampled;
-            uint64_t key;
-            uint2 pq;
+            f3 o, d;
+            PathState ps;
             if (PRIMARY) {
                 p = q;
                 const CameraSample cs = camera_sample(fp, fc, p);
                 o = cs.o;
                 d = cs.d;
-                thr = mk(1, 1, 1);
-                rad = mk(0, 0, 0);
-                cnt = 0;
-                sampled = false;
-                key = path_key(fc.seed, cs.i, cs.j, cs.index);
-                pq = make_uint2(cs.pseed, cs.index);
+                ps.thr = mk(1, 1, 1);
+                ps.rad = mk(0, 0, 0);
+                ps.cnt = 0;
+                ps.sampled = false;
+                ps.key = path_key(fc.seed, cs.i, cs.j, cs.index);
+                ps.pq = make_uint2(cs.pseed, cs.index);
             } else {
                 p = fp.rpath[cur][q];
                 const float4 o4 = fp.ro[cur][q], d4 = fp.rd[cur][q];
                 const float4 A = fp.pa[p], B = fp.pb[p];
                 const uint2 kk = fp.pkey[p];
-                pq = fp.pq[p];
-                key = (static_cast<uint64_t>(kk.y) << 32) | kk.x;
-                thr = xyz(A);
-                rad = xyz(B);
-                cnt = __float_as_uint(A.w);
-                sampled = (__float_as_uint(B.w) & 1u) != 0;
+                ps.pq = fp.pq[p];
+                ps.key = (static_cast<uint64_t>(kk.y) << 32) | kk.x;
+                ps.thr = xyz(A);
+                ps.rad = xyz(B);
+                ps.cnt = __float_as_uint(A.w);
+                ps.sampled = (__float_as_uint(B.w) & 1u) != 0;
                 o = xyz(o4);
                 d = xyz(d4);
             }
-            uint32_t flags = 0;
-            if (h.kind == 0) {
-                // renderer.hpp:24-29
-                f3 bg = mk(0, 0, 0);
-                if (sc.flags & 1u) {
-                    const f3 unit = normalize(d);
-                    const float t = 0.5f * (unit.y + 1);
-                    bg = mk(1, 1, 1) * (1 - t) + mk(0.5f, 0.7f, 1.0f) * t;
-                }
-                rad = rad + mul(thr, bg);
-                if (fp.want_aux && b == 0) {
-                    fp.paux[2 * p] = make_float4(1, 1, 1, 0);
-                    fp.paux[2 * p + 1] = make_float4(0, 0, 0, 0);
-                }
-            } else {
-                // The hit record (geometry.hpp:135-151, sphere.hpp:50-58,
-                // triangle.hpp:63-74, geometry.hpp:231-244).
-                const float t = h.t;
-                const f3 pnt = o + d * t;
-                f3 nrm;
-                bool front;
-                uint32_t mat;
-                bool sampled_light = false;
-                if (h.kind <= 2) {
-                    const float4 cr = sc.spheres[2 * h.prim];
-                    mat = __float_as_uint(sc.spheres[2 * h.prim + 1].x);
-                    const f3 outward = (pnt - xyz(cr)) * (1 / cr.w);
-                    front = dot(d, outward) < 0;
-                    nrm = front ? outward : -outward;
-                    if (h.kind == 2 && fp.nee) {
-                        const float4 em = sc.mats[3 * mat + 1];
-                        sampled_light = em.x > 0 || em.y > 0 || em.z > 0;
-                    }
-                } else {
-                    const float4* tr = sc.tris + 7 * h.prim;
-                    mat = __float_as_uint(tr[0].w);
-                    const f3 n0 = xyz(tr[3]), n1 = xyz(tr[4]), n2 = xyz(tr[5]), fn = xyz(tr[6]);
-                    const float b0 = 1 - h.u - h.v;
-                    const f3 sn = normalize(n0 * b0 + n1 * h.u + n2 * h.v);
-                    if (h.inst == NONE) {
-                        front = dot(d, fn) < 0;
-                        nrm = front ? sn : -sn;
-                    } else {
-                        const float4* in = sc.inst + 7 * h.inst;
-                        const float4 r0 = in[3], r1 = in[4], r2 = in[5];
-                        const f3 ld = mk(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
-                                         r2.x * d.x + r2.y * d.y + r2.z * d.z);
-                        front = dot(ld, fn) < 0;
-                        const f3 ln = front ? sn : -sn;
-                        // the transpose of to_object (transform.hpp:23-27)
-                        nrm = normalize(mk(r0.x * ln.x + r1.x * ln.y + r2.x * ln.z, r0.y * ln.x + r1.y * ln.y + r2.y * ln.z,
-                                           r0.z * ln.x + r1.z * ln.y + r2.z * ln.z));
-                        const uint32_t im = __float_as_uint(in[6].y);
-                        if (im != NONE) mat = im;
-                    }
-                }
-                const float4 m0 = sc.mats[3 * mat], m1 = sc.mats[3 * mat + 1], m2 = sc.mats[3 * mat + 2];
-                const f3 emission = xyz(m1);
-                if (front && !(sampled && sampled_light)) rad = rad + mul(thr, emission);
-
-                // material::scatter (material.hpp:59-137)
-                const int type = static_cast<int>(m0.w);
-                f3 att = xyz(m0), dir = mk(0, 0, 0);
-                bool ok = true;
-                if (type == 0) {
-                    float a, bq;
-                    sample_2d(pq.x, pq.y, 1 + 2 * b, key, cnt, a, bq);
-                    const float r = sqrtf(a), phi = 2 * PI_F * bq;
-                    const f3 nn = nrm;
-                    const float sign = copysignf(1.0f, nn.z);
-                    const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
-                    const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
-                    const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
-                    dir = t1 * (r * cosf(phi)) + t2 * (r * sinf(phi)) + nn * sqrtf(fmaxf(0.0f, 1 - a));
-                } else if (type == 1) {
-                    const f3 reflected = reflect(normalize(d), nrm);
-                    f3 ball = mk(0, 0, 0);
-                    for (int tries = 0; tries < 64; ++tries) {  // vec3.hpp:121-127
-                        const float x = -1 + 2 * rand01(key, cnt);
-                        const float y = -1 + 2 * rand01(key, cnt);
-                        const float z = -1 + 2 * rand01(key, cnt);
-                        ball = mk(x, y, z);
-                        if (dot(ball, ball) < 1) break;
-                    }
-                    dir = reflected + ball * m1.w;
-                    ok = dot(dir, nrm) > 0;
-                } else if (type == 2) {
-                    att = mk(1, 1, 1);
-                    const float ior = m2.x;
-                    const float ratio = front ? (1 / ior) : ior;
-                    const f3 ud = normalize(d);
-                    const float cos_theta = fminf(dot(-ud, nrm), 1.0f);
-                    const float sin_theta = sqrtf(1 - cos_theta * cos_theta);
-                    const bool cannot_refract = ratio * sin_theta > 1;
-                    const bool refl = cannot_refract || schlick(cos_theta, ratio) > rand01(key, cnt);
-                    dir = refl ? reflect(ud, nrm) : refract(ud, nrm, ratio);
-                } else {
-                    ok = false;
-                }
-                if (fp.want_aux && b == 0) {
-                    const f3 fa = ok && type != 2 ? att : mk(1, 1, 1);
-                    fp.paux[2 * p] = make_float4(fa.x, fa.y, fa.z, 0);
-                    fp.paux[2 * p + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0);
-                }
-                if (ok) {
-                    const bool nee = type == 0 && fp.nee && sc.n_lights > 0 && b + 1 < fp.depth;
-                    if (nee) {
-                        // direct_light (renderer.hpp:41-80)
-                        const uint32_t nl = sc.n_lights;
-                        const uint32_t pick = min(static_cast<uint32_t>(rand01(key, cnt) * nl), nl - 1);
-                        const uint32_t ls = sc.lights[pick];
-                        const float4 lcr = sc.spheres[2 * ls];
-                        const uint32_t lmat = __float_as_uint(sc.spheres[2 * ls + 1].x);
-                        const f3 to_center = xyz(lcr) - pnt;
-                        const float dist2 = dot(to_center, to_center), r2 = lcr.w * lcr.w;
-                        float s1, s2;
-                        if (dist2 > r2) {
-                            const float sin2_max = r2 / dist2;
-                            const float cos_max = sqrtf(1 - sin2_max);
-                            const float omc = sin2_max / (1 + cos_max);
-                            const f3 w = to_center * (1 / sqrtf(dist2));
-                            const f3 ax = fabsf(w.x) > 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
-                            const f3 uu = normalize(cross(w, ax)), vv = cross(w, uu);
-                            sample_2d(pq.x, pq.y, 2 + 2 * b, key, cnt, s1, s2);
-                            const float cos_t = 1 - s1 * omc;
-                            const float sin_t = sqrtf(fmaxf(0.0f, 1 - cos_t * cos_t));
-                            const float phi = 2 * PI_F * s2;
-                            const f3 ldir = uu * (cosf(phi) * sin_t) + vv * (sinf(phi) * sin_t) + w * cos_t;
-                            const float cos_surface = dot(ldir, nrm);
-                            float t_light = __builtin_huge_valf();
-                            if (cos_surface > 0 && sphere_hit_f(lcr, pnt, ldir, 0.001f, t_light)) {
-                                const f3 lp = pnt + ldir * t_light;
-                                const f3 outward = (lp - xyz(lcr)) * (1 / lcr.w);
-                                if (dot(ldir, outward) < 0) {
-                                    const float pdf = 1 / (2 * PI_F * omc);
-                                    const f3 contrib = xyz(sc.mats[3 * lmat + 1]) *
-                                                       (cos_surface / PI_F / pdf * static_cast<float>(nl));
-                                    shadow = true;
-                                    sh_d = ldir;
-                                    sh_tmax = t_light * 0.9999f;
-                                    sh_l = mul(mul(thr, att), contrib);
-                                } else {
-                                    backface = true;
-                                }
-                            }
-                        }
-                    }
-                    flags = nee ? 1u : 0u;
-                    thr = mul(thr, att);
-                    bool alive = true;
-                    if (b + 1 >= 3) {  // roulette_after (renderer.hpp:22,140-146)
-                        const float pm = fmaxf(thr.x, fmaxf(thr.y, thr.z));
-                        if (pm < 1) {
-                            if (rand01(key, cnt) >= pm) alive = false;
-                            else thr = thr * (1 / pm);
-                        }
-                    }
-                    if (alive && b + 1 < fp.depth) {
-                        cont = true;
-                        next_o = pnt;
-                        next_d = dir;
-                    }
-                }
-                next_o = pnt;
-            }
+            r = shade_hit(fp, b, p, h, o, d, ps);
             // The radiance always (accumulate reads it); the rest only for a
             // path that goes on.
-            fp.pb[p] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(flags));
-            if (cont) {
-                fp.pa[p] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(cnt));
+            fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, __uint_as_float(ps.sampled ? 1u : 0u));
+            if (r.cont) {
+                fp.pa[p] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, __uint_as_float(ps.cnt));
                 if (PRIMARY) {
-                    fp.pkey[p] = make_uint2(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32));
-                    fp.pq[p] = pq;
+                    fp.pkey[p] = make_uint2(static_cast<uint32_t>(ps.key), static_cast<uint32_t>(ps.key >> 32));
+                    fp.pq[p] = ps.pq;
                 }
             }
         }
         // Every thread of the block reaches the appends.
-        const bool pred[3] = {cont, shadow, backface};
+        const bool pred[3] = {r.cont, r.shadow, r.backface};
         uint32_t slot[3];
         block_append<3>(sh, counters, pred, slot);
-        if (cont) {
-            fp.ro[nxt][slot[0]] = make_float4(next_o.x, next_o.y, next_o.z, T_MIN);
-            fp.rd[nxt][slot[0]] = make_float4(next_d.x, next_d.y, next_d.z, T_MAX);
+        if (r.cont) {
+            fp.ro[nxt][slot[0]] = make_float4(r.pnt.x, r.pnt.y, r.pnt.z, T_MIN);
+            fp.rd[nxt][slot[0]] = make_float4(r.dir.x, r.dir.y, r.dir.z, T_MAX);
             fp.rpath[nxt][slot[0]] = p;
         }
-        if (shadow) {
-            fp.so[slot[1]] = make_float4(next_o.x, next_o.y, next_o.z, 0.001f);
-            fp.sd[slot[1]] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_tmax);
-            fp.sl[slot[1]] = make_float4(sh_l.x, sh_l.y, sh_l.z, __uint_as_float(p));
+        if (r.shadow) {
+            fp.so[slot[1]] = make_float4(r.pnt.x, r.pnt.y, r.pnt.z, 0.001f);
+            fp.sd[slot[1]] = make_float4(r.sh_d.x, r.sh_d.y, r.sh_d.z, r.sh_tmax);
+            fp.sl[slot[1]] = make_float4(r.sh_l.x, r.sh_l.y, r.sh_l.z, __uint_as_float(p));
         }
+    }
+}
+
+// Whether bounce b starts the tail: the queues left (rays of bounce b and
+// the light samples of bounce b - 1) are too small to fill the GPU stage by
+// stage, and no earlier bounce started it.
+__device__ __forceinline__ bool tail_starts(const FrameParams& fp, int b) {
+    if (fp.frame->tail_from < static_cast<uint32_t>(b)) return false;
+    return fp.ctrl[b - 1].n_shadow + fp.ctrl[b].n_rays < fp.tail_threshold;
+}
+
+// Tail, part 1: bounce b - 1's light samples, so that their radiance is in
+// place before part 2 carries the paths on.
+template <int S, bool INST>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_tail_shadow(FrameParams fp, int b) {
+    if (!tail_starts(fp, b)) return;
+    __shared__ uint2 lds[S * TRACE_BLOCK];
+    auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
+    const uint32_t n = fp.ctrl[b - 1].n_shadow;
+    uint32_t* work = &fp.ctrl[MAX_BOUNCES].w_unused;
+    const uint32_t groups = fetch_groups(n);
+    while (true) {
+        const uint32_t base = wave_fetch(work, 64 * groups);
+        if (base >= n) break;
+        for (uint32_t g = 0; g < groups; ++g) {
+            const uint32_t item = base + 64 * g + lane_id();
+            if (item >= n) break;
+            const float4 a = fp.so[item], bb = fp.sd[item];
+            Hit h;
+            if (!trace_ray<true, false, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
+                const float4 l = fp.sl[item];
+                const uint32_t p = __float_as_uint(l.w);
+                float4 r = fp.pb[p];
+                r.x = r.x + l.x;
+                r.y = r.y + l.y;
+                r.z = r.z + l.z;
+                fp.pb[p] = r;
+            }
+        }
+    }
+}
+
+// Tail, part 2: every path still going at bounce b taken to its end, lane by
+// lane: closest hit, shade, light sample, next bounce. The later bounces'
+// stage launches then return at once. Rays are counted as the stages count
+// them, into ctrl[MAX_BOUNCES].
+template <int S, bool INST>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
+    if (!tail_starts(fp, b)) return;
+    Ctrl* c = fp.ctrl + b;
+    const uint32_t n = c->n_rays;
+    if (blockIdx.x == 0 && threadIdx.x == 0) fp.frame->tail_from = b;  // read by later launches only
+    __shared__ uint2 lds[S * TRACE_BLOCK];
+    auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
+    uint32_t rays = 0, shadows = 0, backfaces = 0;
+    const int cur = b & 1;
+    const uint32_t groups = fetch_groups(n);
+    while (true) {
+        const uint32_t base = wave_fetch(&c->w_extend, 64 * groups);
+        if (base >= n) break;
+        for (uint32_t g = 0; g < groups; ++g) {
+            const uint32_t q = base + 64 * g + lane_id();
+            if (q >= n) break;
+            const uint32_t p = fp.rpath[cur][q];
+            const float4 o4 = fp.ro[cur][q], d4 = fp.rd[cur][q];
+            const float4 A = fp.pa[p], B = fp.pb[p];
+            const uint2 kk = fp.pkey[p];
+            PathState ps;
+            ps.pq = fp.pq[p];
+            ps.key = (static_cast<uint64_t>(kk.y) << 32) | kk.x;
+            ps.thr = xyz(A);
+            ps.rad = xyz(B);
+            ps.cnt = __float_as_uint(A.w);
+            ps.sampled = (__float_as_uint(B.w) & 1u) != 0;
+            f3 o = xyz(o4), d = xyz(d4);
+            float tmin = o4.w, tmax = d4.w;
+            for (int bb = b; bb < fp.depth; ++bb) {
+                Hit h;
+                trace_ray<false, false, INST>(fp.sc, o, d, tmin, tmax, h, st);
+                if (bb > b) ++rays;  // bounce b's rays were counted when they were queued
+                const ShadeResult r = shade_hit(fp, bb, p, h, o, d, ps);
+                if (r.backface) ++backfaces;
+                if (r.shadow) {
+                    ++shadows;
+                    Hit hs;
+                    if (!trace_ray<true, false, INST>(fp.sc, r.pnt, r.sh_d, 0.001f, r.sh_tmax, hs, st)) {
+                        ps.rad.x = ps.rad.x + r.sh_l.x;
+                        ps.rad.y = ps.rad.y + r.sh_l.y;
+                        ps.rad.z = ps.rad.z + r.sh_l.z;
+                    }
+                }
+                if (!r.cont) break;
+                o = r.pnt;
+                d = r.dir;
+                tmin = T_MIN;
+                tmax = T_MAX;
+            }
+            fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, 0.0f);
+        }
+    }
+    // Per wave, one atomic per count.
+    uint32_t v[3] = {rays, shadows, backfaces};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    if (lane_id() == 0) {
+        Ctrl* t = fp.ctrl + MAX_BOUNCES;
+        if (v[0]) atomicAdd(&t->n_rays, v[0]);
+        if (v[1]) atomicAdd(&t->n_shadow, v[1]);
+        if (v[2]) atomicAdd(&t->n_backface, v[2]);
     }
 }
 
@@ -475,11 +608,16 @@ __global__ void k_finish(FrameParams fp) {
         rays += fp.ctrl[b].n_rays;
         shadow += fp.ctrl[b].n_shadow + fp.ctrl[b].n_backface;
     }
+    const Ctrl& t = fp.ctrl[MAX_BOUNCES];  // the tail launch's counts
+    rays += t.n_rays;
+    shadow += t.n_shadow + t.n_backface;
     fp.totals[0] += rays;
     fp.totals[1] += shadow;
     fp.totals[2] += fp.ctrl[0].n_rays;
     for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
+    fp.ctrl[MAX_BOUNCES] = Ctrl{};
     fp.frame->batch += 1;
+    fp.frame->tail_from = ~0u;
 }
 
 __global__ __launch_bounds__(256) void k_clear_frame(FrameParams fp) {
@@ -493,8 +631,10 @@ __global__ __launch_bounds__(256) void k_clear_frame(FrameParams fp) {
     }
     if (px == 0) {
         fp.frame->batch = 0;
+        fp.frame->tail_from = ~0u;
         fp.totals[0] = fp.totals[1] = fp.totals[2] = 0;
         for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
+        fp.ctrl[MAX_BOUNCES] = Ctrl{};
     }
 }
 
@@ -524,6 +664,10 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams fp, float* rgb, flo
 template <int S, bool INST>
 static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev) {
     for (int b = 0; b < fp.depth; ++b) {
+        if (b > 0) {
+            k_tail_shadow<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            k_tail<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+        }
         if (ev) (void)hipEventRecord(ev[2 * b], stream);
         if (b == 0) {
             if (fp.visits) k_extend<S, true, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);

@@ -76,7 +76,7 @@ struct trace_occupancy {
 // slab tests, leaf references, triangle tests, sphere tests, instance
 // entries, large-sphere tests. They price a query's algorithmic bytes.
 struct Visits {
-    uint32_t nodes = 0, leafrefs = 0, tris = 0, spheres = 0, instances = 0, large = 0;
+    uint32_t nodes = 0, leafrefs = 0, tris = 0, spheres = 0, instances = 0, large = 0, queries = 0;
 };
 constexpr int COUNT_SHIFT = 28;
 
@@ -151,6 +151,7 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
                                           Stack& st, Visits* vis = nullptr) {
     uint32_t kind = 0, prim = 0, inst = NONE;
     float hu = 0, hv = 0;
+    if (COUNT) ++vis->queries;
 
     // Large spheres, in double (geometry.hpp:117-118, 157-160).
     for (uint32_t k = 0; k < sc.n_large; ++k) {
@@ -299,9 +300,9 @@ __device__ __forceinline__ LaneStack<S, BLOCK> lane_stack(uint2* lds, uint2* spi
 
 // Adds a lane's visit counts into six 64-bit totals, one atomic per wave and counter.
 __device__ __forceinline__ void add_visits(unsigned long long* totals, const Visits& v) {
-    const uint32_t x[6] = {v.nodes, v.leafrefs, v.tris, v.spheres, v.instances, v.large};
+    const uint32_t x[7] = {v.nodes, v.leafrefs, v.tris, v.spheres, v.instances, v.large, v.queries};
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 7; ++k) {
         unsigned long long s = x[k];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);

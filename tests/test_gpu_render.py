@@ -49,6 +49,18 @@ def test_determinism_and_batching():
     assert not np.array_equal(a, e)
 
 
+@pytest.mark.parametrize("name", ["field", "room", "crowd", "config1"])
+def test_tail_launch_equals_stage_by_stage(name):
+    """Finishing the last, small queues path by path in one launch gives the
+    same bytes and ray counts as running every bounce stage by stage."""
+    ptgp = scene_of(name)
+    with pt.Scene(ptgp) as s:
+        a, _, sa = s.render(160, 90, 16, seed=21)
+        b, _, sb = s.render(160, 90, 16, seed=21, tail=False)
+    assert np.array_equal(a, b)
+    assert (sa.rays, sa.shadow_rays, sa.primary_rays) == (sb.rays, sb.shadow_rays, sb.primary_rays)
+
+
 def test_device_output_equals_host_output():
     ptgp = scene_of("field")
     with pt.Scene(ptgp) as s:
