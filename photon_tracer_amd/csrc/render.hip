@@ -130,7 +130,7 @@ __device__ __forceinline__ CameraSample camera_sample(const FrameParams& fp, con
 // whose rays are the batch's camera rays, made here; it has no shadow rays.
 template <int S, bool COUNT, bool INST, bool PRIMARY>
 __global__ __launch_bounds__(TRACE_BLOCK)
-__attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST>::waves))) void k_extend(FrameParams fp, int b) {
+__attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, !PRIMARY>::waves))) void k_extend(FrameParams fp, int b) {
     if (!PRIMARY && fp.frame->tail_from <= static_cast<uint32_t>(b)) return;  // the tail launch has these rays
     __shared__ uint2 lds[S * TRACE_BLOCK];
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
@@ -567,37 +567,68 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
 }
 
 // Per pixel, the batch's samples added in sample order onto the running sum
-// (renderer.hpp:379-396).
+// (renderer.hpp:379-396). A pixel's samples are adjacent (spb float4), so a
+// block first copies a tile of pixels' samples into LDS with coalesced loads,
+// then one thread per pixel adds them up in order. The aux sums, and batches
+// with more samples than a tile holds, take the direct loop.
+constexpr uint32_t ACC_TILE = 1024;  // float4 per tile
+
 __global__ __launch_bounds__(256) void k_accumulate(FrameParams fp) {
-    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= fp.npix) return;
-    float4 acc = fp.accum[px];
-    f3 sum = xyz(acc);
-    float4 a0 = make_float4(0, 0, 0, 0), a1 = make_float4(0, 0, 0, 0);
-    if (fp.want_aux) {
-        a0 = fp.aux_accum[2 * px];
-        a1 = fp.aux_accum[2 * px + 1];
-    }
-    for (int s = 0; s < fp.spb; ++s) {
-        const uint32_t p = px * fp.spb + s;
-        const f3 c = xyz(fp.pb[p]);
-        sum = sum + c;
-        if (fp.want_aux) {
-            const float4 fa = fp.paux[2 * p], fn = fp.paux[2 * p + 1];
-            a0.x = a0.x + fa.x;
-            a0.y = a0.y + fa.y;
-            a0.z = a0.z + fa.z;
-            a1.x = a1.x + fn.x;
-            a1.y = a1.y + fn.y;
-            a1.z = a1.z + fn.z;
-            const float lum = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;  // renderer.hpp:364
-            a0.w += lum * lum;
+    // A pixel's row in the tile is padded by one float4 when spb >= 8 (then
+    // at most ACC_TILE / 8 pixels per tile), so the adders, reading at a
+    // stride of spb float4, spread over the LDS banks.
+    __shared__ float4 tile[ACC_TILE + ACC_TILE / 8];
+    const uint32_t spb = static_cast<uint32_t>(fp.spb);
+    if (!fp.want_aux && spb <= ACC_TILE) {
+        const uint32_t tp = ACC_TILE / spb;                 // pixels per tile
+        const uint32_t pitch = spb + (spb >= 8 ? 1u : 0u);
+        for (uint32_t px0 = blockIdx.x * tp; px0 < fp.npix; px0 += gridDim.x * tp) {
+            const uint32_t np = min(tp, fp.npix - px0);
+            const uint32_t count = np * spb;
+            const float4* src = fp.pb + static_cast<size_t>(px0) * spb;
+            for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) tile[(k / spb) * pitch + k % spb] = src[k];
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < np; q += blockDim.x) {
+                const uint32_t px = px0 + q;
+                const float4 acc = fp.accum[px];
+                f3 sum = xyz(acc);
+                const float4* t = tile + q * pitch;
+                for (uint32_t s = 0; s < spb; ++s) sum = sum + xyz(t[s]);
+                fp.accum[px] = make_float4(sum.x, sum.y, sum.z, 0);
+            }
+            __syncthreads();
         }
+        return;
     }
-    fp.accum[px] = make_float4(sum.x, sum.y, sum.z, 0);
-    if (fp.want_aux) {
-        fp.aux_accum[2 * px] = a0;
-        fp.aux_accum[2 * px + 1] = a1;
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < fp.npix; px += gridDim.x * blockDim.x) {
+        float4 acc = fp.accum[px];
+        f3 sum = xyz(acc);
+        float4 a0 = make_float4(0, 0, 0, 0), a1 = make_float4(0, 0, 0, 0);
+        if (fp.want_aux) {
+            a0 = fp.aux_accum[2 * px];
+            a1 = fp.aux_accum[2 * px + 1];
+        }
+        for (uint32_t s = 0; s < spb; ++s) {
+            const uint32_t p = px * spb + s;
+            const f3 c = xyz(fp.pb[p]);
+            sum = sum + c;
+            if (fp.want_aux) {
+                const float4 fa = fp.paux[2 * p], fn = fp.paux[2 * p + 1];
+                a0.x = a0.x + fa.x;
+                a0.y = a0.y + fa.y;
+                a0.z = a0.z + fa.z;
+                a1.x = a1.x + fn.x;
+                a1.y = a1.y + fn.y;
+                a1.z = a1.z + fn.z;
+                const float lum = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;  // renderer.hpp:364
+                a0.w += lum * lum;
+            }
+        }
+        fp.accum[px] = make_float4(sum.x, sum.y, sum.z, 0);
+        if (fp.want_aux) {
+            fp.aux_accum[2 * px] = a0;
+            fp.aux_accum[2 * px + 1] = a1;
+        }
     }
 }
 
@@ -663,6 +694,7 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams fp, float* rgb, flo
 
 template <int S, bool INST>
 static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev) {
+    const int acc_grid = shade_grid * 8;
     for (int b = 0; b < fp.depth; ++b) {
         if (b > 0) {
             k_tail_shadow<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
@@ -682,7 +714,7 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
     }
     // No light samples are taken at the last bounce (renderer.hpp:127-130),
     // so no shadow rays are left over.
-    k_accumulate<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp);
+    k_accumulate<<<acc_grid, 256, 0, stream>>>(fp);
     k_finish<<<1, 64, 0, stream>>>(fp);
 }
 

@@ -66,10 +66,12 @@ constexpr uint32_t FIRST_MASK = 0x0fffffffu;
 
 // Waves per SIMD the traversal kernels are compiled for. The 8-entry ring
 // without the instance walk fits 64 VGPRs (8 waves, 16 KB of LDS per block
-// of 4 waves) with no scratch; the others keep the compiler's own budget.
-template <int S, bool INST>
+// of 4 waves); the others keep the compiler's own budget. A kernel that also
+// carries the shadow-ray loop (MERGED) gets 7 waves: at 64 VGPRs it spills
+// 28 bytes a lane to scratch, at 72 it spills nothing and runs faster.
+template <int S, bool INST, bool MERGED = false>
 struct trace_occupancy {
-    static constexpr int waves = (S == 8 && !INST) ? 8 : 1;
+    static constexpr int waves = (S == 8 && !INST) ? (MERGED ? 7 : 8) : 1;
 };
 
 // Visit counts of one query, kept only by the counting kernel variants: node
