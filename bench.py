@@ -15,6 +15,9 @@ Besides the timed steps it reports:
                 4 per leaf reference, 48 per triangle, 16 per sphere, 52 per
                 instance, 16 per large sphere; visit counts from a counting
                 render of the same rays) over its HIP-event time per launch;
+                "traffic" is the HBM bytes per launch that rocprofv3's
+                FETCH_SIZE and WRITE_SIZE counters read for the same launches
+                (profiles/, default workload only);
   cpu_baseline  the reference's own renderer (oracle/_ref, its headers
                 compiled with its flags) on the host cores, on a bounded
                 sample of the same frame.
@@ -32,6 +35,22 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s (1 GPU); equal-time relMSE guided vs plain NEE, 1080p 1M tris"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per traversal-stage launch on the default workload, measured by
+# two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over this script; the
+# file records the commands and the copy-kernel calibration of each counter.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_extend.icosphere_1M_1080p_64spp.hbm_traffic.txt")
+
+
+def measured_traffic():
+    """hbm_bytes_per_launch from the counter summary, or None without one."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch="):
+                    return int(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -150,10 +169,12 @@ def main():
         queries = cst.visits[6]  # closest-hit and shadow queries traced by the traversal-stage launches
         algo = pt.QUERY_BYTES * queries + sum(v * b for v, b in zip(cst.visits, pt.VISIT_BYTES))
         launches = 10 * tst.batches
+        default_workload = (W, H, spp, args.scale, args.stack) == (1920, 1080, 64, 1.0, 0)
+        traffic = measured_traffic() if default_workload else None
         achieved = algo / (tst.extend_ms * 1e-3) / 1e9
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_extend (traversal stage: closest-hit rays + the previous bounce's shadow rays)",
             "launches": launches, "avg_launch_ms": round(tst.extend_ms / launches, 4),
             "algorithmic_bytes_per_launch": int(algo / launches),

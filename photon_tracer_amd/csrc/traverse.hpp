@@ -41,14 +41,19 @@ struct LaneStack {
     int top, base;     // entries [0, base) are in HBM, [base, top) in the ring
 
     __device__ __forceinline__ void reset() { top = base = 0; }
-    __device__ __forceinline__ void push(uint32_t code, float t) {
-        if (top - base == S) {
+    // A node's hit children are pushed together: reserve() makes room for m
+    // entries (the oldest move out to HBM), put() writes entry top + k,
+    // advance() counts the m entries in.
+    __device__ __forceinline__ void reserve(int m) {
+        while (top - base + m > S) {
             __builtin_nontemporal_store(lds[(base & (S - 1)) * BLOCK], spill + static_cast<size_t>(base) * stride);
             ++base;
         }
-        lds[(top & (S - 1)) * BLOCK] = pack_entry(code, t);
-        ++top;
     }
+    __device__ __forceinline__ void put(int k, uint32_t code, float t) {
+        lds[((top + k) & (S - 1)) * BLOCK] = pack_entry(code, t);
+    }
+    __device__ __forceinline__ void advance(int m) { top += m; }
     __device__ __forceinline__ uint2 pop() {
         --top;
         unsigned long long v = lds[(top & (S - 1)) * BLOCK];   // the ring slot, read unconditionally
@@ -66,12 +71,13 @@ constexpr uint32_t FIRST_MASK = 0x0fffffffu;
 
 // Waves per SIMD the traversal kernels are compiled for. The 8-entry ring
 // without the instance walk fits 64 VGPRs (8 waves, 16 KB of LDS per block
-// of 4 waves); the others keep the compiler's own budget. A kernel that also
-// carries the shadow-ray loop (MERGED) gets 7 waves: at 64 VGPRs it spills
-// 28 bytes a lane to scratch, at 72 it spills nothing and runs faster.
-template <int S, bool INST, bool MERGED = false>
+// of 4 waves); the others keep the compiler's own budget. The path loop's
+// traversal stage (REFILL: lane refill, camera rays or queued rays and light
+// samples) gets 7 waves: at 64 VGPRs it spills to scratch, at 72 it does not
+// and runs faster.
+template <int S, bool INST, bool REFILL = false>
 struct trace_occupancy {
-    static constexpr int waves = (S == 8 && !INST) ? (MERGED ? 7 : 8) : 1;
+    static constexpr int waves = (S == 8 && !INST) ? (REFILL ? 7 : 8) : 1;
 };
 
 // Visit counts of one query, kept only by the counting kernel variants: node
@@ -113,33 +119,31 @@ __device__ __forceinline__ bool visit_node(const Node* __restrict__ nodes, uint3
     }
     // Position of each hit child in the farthest-first order: those entered
     // later come first, and on equal entry the lower child number first.
-    int rank[4];
+    int rank[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        int r = 0;
+    for (int a = 0; a < 4; ++a) {
 #pragma unroll
-        for (int c2 = 0; c2 < 4; ++c2) {
-            if (c2 == c) continue;
-            const bool before = tn[c2] > tn[c] || (tn[c2] == tn[c] && c2 < c);
-            r += (h[c2] && before) ? 1 : 0;
+        for (int b2 = a + 1; b2 < 4; ++b2) {
+            const bool later = tn[b2] > tn[a];     // b2 is pushed before a
+            rank[a] += (h[b2] && later) ? 1 : 0;
+            rank[b2] += (h[a] && !later) ? 1 : 0;
         }
-        rank[c] = r;
     }
     const int n = (h[0] ? 1 : 0) + (h[1] ? 1 : 0) + (h[2] ? 1 : 0) + (h[3] ? 1 : 0);
+    if (n == 0) return false;
+    st.reserve(n - 1);
+    uint32_t ncd = 0;
+    float nt = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        uint32_t cd = 0;
-        float t = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bool sel = h[c] && rank[c] == k;
-            cd = sel ? code[c] : cd;
-            t = sel ? tn[c] : t;
-        }
-        if (k < n - 1) st.push(cd, t);
-        else if (k == n - 1) next = make_uint2(cd, __float_as_uint(t));
+    for (int c = 0; c < 4; ++c) {
+        if (h[c] && rank[c] < n - 1) st.put(rank[c], code[c], tn[c]);
+        const bool sel = h[c] && rank[c] == n - 1;
+        ncd = sel ? code[c] : ncd;
+        nt = sel ? tn[c] : nt;
     }
-    return n > 0;
+    st.advance(n - 1);
+    next = make_uint2(ncd, __float_as_uint(nt));
+    return true;
 }
 
 __device__ __forceinline__ f3 safe_rcp3(f3 d) { return {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
@@ -286,6 +290,88 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
     hit.prim = prim;
     hit.inst = inst;
     return kind != 0;
+}
+
+// One ray's walk of the flat tree, advanced one stack entry per step, so a
+// lane whose ray has ended can start another while the rest of its wave is
+// still walking. A ray goes through exactly the steps trace_ray takes for it
+// (scenes without instances), with the same results.
+struct Walk {
+    f3 o, d, inv;
+    float tmin, tmax, hu, hv;
+    uint32_t kind, prim;
+    uint2 e;       // the entry to take next when `have`
+    bool have;
+};
+
+// Starts a walk: the large spheres in double, then the flat tree's root.
+// Returns true if the query has already ended (an any-hit query that hit a
+// large sphere).
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool walk_begin(const SceneView& sc, Walk& w, f3 o, f3 d, float tmin, float tmax,
+                                           bool any, Stack& st, Visits* vis) {
+    if (COUNT) ++vis->queries;
+    w.o = o;
+    w.d = d;
+    w.tmin = tmin;
+    w.tmax = tmax;
+    w.hu = w.hv = 0;
+    w.kind = w.prim = 0;
+    for (uint32_t k = 0; k < sc.n_large; ++k) {
+        if (COUNT) ++vis->large;
+        const uint32_t s = sc.large[k];
+        if (sphere_hit_d(sc.spheres[2 * s], o, d, tmin, w.tmax)) {
+            w.kind = 1;
+            w.prim = s;
+            if (any) return true;
+        }
+    }
+    w.inv = safe_rcp3(d);
+    st.reset();
+    w.e = make_uint2(sc.flat_root, __float_as_uint(tmin));
+    w.have = sc.flat_root != NONE;
+    return false;
+}
+
+// One stack entry of the walk: a node's slab test, or a leaf's primitives.
+// Returns true when the walk has ended (the stack is empty, or an any-hit
+// query found a hit); kind != 0 then says something was hit.
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool walk_step(const SceneView& sc, Walk& w, bool any, Stack& st, Visits* vis) {
+    if (!w.have) {
+        if (st.top == 0) return true;
+        w.e = st.pop();
+    }
+    w.have = false;
+    if (__uint_as_float(w.e.y) >= w.tmax) return false;
+    const uint32_t first = w.e.x & FIRST_MASK, count = w.e.x >> COUNT_SHIFT;
+    if (count == 0) {
+        if (COUNT) ++vis->nodes;
+        w.have = visit_node(sc.nodes, first, w.o, w.inv, w.tmin, w.tmax, st, w.e);
+        return false;
+    }
+    for (uint32_t i = first; i < first + count; ++i) {
+        const float4 r0 = sc.leaf[3 * i], r1 = sc.leaf[3 * i + 1];
+        if (COUNT) {
+            ++vis->leafrefs;
+            if (__float_as_uint(r1.w) != 0u) ++vis->spheres;
+            else ++vis->tris;
+        }
+        bool hit;
+        if (__float_as_uint(r1.w) != 0u) {
+            hit = sphere_hit_f(make_float4(r0.x, r0.y, r0.z, r1.x), w.o, w.d, w.tmin, w.tmax);
+            if (hit) w.kind = 2;
+        } else {
+            const float4 r2 = sc.leaf[3 * i + 2];
+            hit = tri_hit(xyz(r0), xyz(r1), xyz(r2), w.o, w.d, w.tmin, w.tmax, w.hu, w.hv);
+            if (hit) w.kind = 3;
+        }
+        if (hit) {
+            w.prim = __float_as_uint(r0.w);
+            if (any) return true;
+        }
+    }
+    return false;
 }
 
 // One lane's stack over the block's LDS ring array and the grid's spill area.
