@@ -160,7 +160,11 @@ enum {
     PT_RENDER_TIMING = 4,        /* eager, with HIP events around every traversal launch */
     PT_RENDER_COUNT = 8,         /* eager, the traversal stages count their visits (stats->visits) */
     PT_RENDER_DEVICE_OUTPUT = 16,/* rgb and aux are device addresses (pt_device_alloc): no copy to the host */
-    PT_RENDER_NO_TAIL = 32       /* every bounce stage by stage: no tail launch for the last, small queues */
+    PT_RENDER_NO_TAIL = 32,      /* every bounce stage by stage: no tail launch for the last, small queues */
+    PT_RENDER_GUIDE = 64,        /* diffuse bounces sample the attached guide (pt_scene_set_guide) 50/50
+                                    with the cosine lobe, one-sample MIS */
+    PT_RENDER_TRAIN = 128        /* also record each path's diffuse vertices and, after every batch, take
+                                    Adam steps on the guide over them (implies PT_RENDER_GUIDE) */
 };
 
 /* Render statistics. */
@@ -192,6 +196,56 @@ pt_status pt_device_alloc(size_t bytes, void** ptr);
 pt_status pt_device_free(void* ptr);
 /* kind: 0 device to host, 1 host to device, 2 device to device. */
 pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int kind);
+
+/* ---------------------------------------------------------------------------
+ * Learned path guide (the config-4 guided tracer). The reference has no path
+ * guiding: these entry points have no counterpart there. The guide is an MLP
+ * (one-blob position 3x16 + normal angles 2x8 = 64 inputs, four hidden layers
+ * of 64 with ReLU, 32 outputs = 8 von Mises-Fisher lobes x (polar logit,
+ * azimuth logit, log kappa, weight logit)), FP16 weights with FP32
+ * accumulation on the GPU's matrix cores, trained by Adam on the tracer's own
+ * radiance samples. A diffuse bounce then samples it 50/50 with the cosine
+ * lobe and weighs by the mixed pdf (one-sample MIS), so images stay unbiased.
+ * ------------------------------------------------------------------------- */
+typedef struct pt_guide pt_guide;
+
+#define PT_GUIDE_PARAMS 18720   /* FP32 parameters: 4 x (64x64 + 64) + 32x64 + 32 */
+#define PT_GUIDE_OUTPUTS 32
+
+typedef struct {
+    uint64_t seed;              /* weight initialisation */
+    float learning_rate;        /* Adam step size; 0 means 1e-3 */
+    float box_lo[3], box_hi[3]; /* position box of the input encoding; all zero: the scene's tree bounds */
+    int32_t steps_per_batch;    /* PT_RENDER_TRAIN: Adam steps after each batch; 0 means one pass over its records */
+    int32_t minibatch;          /* samples per Adam step; 0 means 16384 */
+    float weight_clamp;         /* training weights are clamped at this many times their batch mean; 0 means 16 */
+    int32_t bounces;            /* diffuse vertices of bounces [0, bounces) sample the guide (later ones the
+                                   cosine lobe alone, and the tail launch may take them); 0 means 2 */
+} pt_guide_params;
+
+pt_status pt_guide_create(pt_scene* scene, const pt_guide_params* params, pt_guide** guide);
+pt_status pt_guide_destroy(pt_guide* guide);
+/* Attaches a guide to a scene for PT_RENDER_GUIDE / PT_RENDER_TRAIN renders (NULL detaches). */
+pt_status pt_scene_set_guide(pt_scene* scene, pt_guide* guide);
+/* The FP32 parameters, layer by layer: weights [out][in] row-major, then biases. */
+pt_status pt_guide_get_weights(pt_guide* guide, float* params, size_t n);
+/* Sets the parameters and clears Adam's moments and step count. */
+pt_status pt_guide_set_weights(pt_guide* guide, const float* params, size_t n);
+/* Network outputs at n shading points (pos, nrm: 3 floats each): 32 floats each. */
+pt_status pt_guide_eval(pt_guide* guide, const float* pos, const float* nrm, size_t n, float* out);
+/* Mixture pdf (per steradian) of direction dir[i] (3 floats) at point i. */
+pt_status pt_guide_pdf(pt_guide* guide, const float* pos, const float* nrm, const float* dir, size_t n, float* pdf);
+/* One mixture sample per point from u[i] = 3 uniforms: direction and pdf (4 floats each). */
+pt_status pt_guide_sample(pt_guide* guide, const float* pos, const float* nrm, const float* u, size_t n,
+                          float* dir_pdf);
+/* Training samples are 12 floats: position, weight, normal, 0, direction, 0; a sample's loss is
+ * weight * -log pdf(direction). pt_guide_gradient gives the mean gradient over n samples (and the
+ * mean loss); pt_guide_step takes one Adam step with it. */
+pt_status pt_guide_gradient(pt_guide* guide, const float* samples, size_t n, float* grad, float* loss);
+pt_status pt_guide_step(pt_guide* guide, const float* samples, size_t n, float* loss);
+/* Training statistics of PT_RENDER_TRAIN renders since the last call: Adam steps taken, samples
+ * used, mean loss of the last step. */
+pt_status pt_guide_stats(pt_guide* guide, uint64_t* steps, uint64_t* samples, float* last_loss);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,10 @@ RENDER_TIMING = 4
 RENDER_COUNT = 8
 RENDER_DEVICE_OUTPUT = 16
 RENDER_NO_TAIL = 32
+RENDER_GUIDE = 64
+RENDER_TRAIN = 128
+GUIDE_PARAMS = 18720
+GUIDE_OUTPUTS = 32
 
 # Algorithmic bytes of one traversal query (the ray in, the hit out, and every
 # record the reference's walk must read): 32 + 16 per query, 128 per node slab
@@ -67,6 +71,12 @@ class _RenderStats(ctypes.Structure):
         ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
         ("visits", ctypes.c_uint64 * 7), ("batches", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
     ]
+
+
+class _GuideParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float), ("box_lo", ctypes.c_float * 3),
+                ("box_hi", ctypes.c_float * 3), ("steps_per_batch", ctypes.c_int32), ("minibatch", ctypes.c_int32),
+                ("weight_clamp", ctypes.c_float), ("bounces", ctypes.c_int32)]
 
 
 class _JobDesc(ctypes.Structure):
@@ -109,8 +119,23 @@ def library() -> ctypes.CDLL:
     lib.pt_device_alloc.argtypes = [sz, ctypes.POINTER(vp)]
     lib.pt_device_free.argtypes = [vp]
     lib.pt_device_copy.argtypes = [vp, vp, sz, ctypes.c_int]
-    for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate", "pt_generate_job", "pt_pack", "pt_ptgp_depth", "pt_scene_create",
-                 "pt_scene_destroy", "pt_trace", "pt_trace_device", "pt_render"):
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.pt_guide_create.argtypes = [vp, ctypes.POINTER(_GuideParams), ctypes.POINTER(vp)]
+    lib.pt_guide_destroy.argtypes = [vp]
+    lib.pt_scene_set_guide.argtypes = [vp, vp]
+    lib.pt_guide_get_weights.argtypes = [vp, vp, sz]
+    lib.pt_guide_set_weights.argtypes = [vp, vp, sz]
+    lib.pt_guide_eval.argtypes = [vp, vp, vp, sz, vp]
+    lib.pt_guide_pdf.argtypes = [vp, vp, vp, vp, sz, vp]
+    lib.pt_guide_sample.argtypes = [vp, vp, vp, vp, sz, vp]
+    lib.pt_guide_gradient.argtypes = [vp, vp, sz, vp, fp]
+    lib.pt_guide_step.argtypes = [vp, vp, sz, fp]
+    lib.pt_guide_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), fp]
+    for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate",
+                 "pt_generate_job", "pt_pack", "pt_ptgp_depth", "pt_scene_create", "pt_scene_destroy", "pt_trace",
+                 "pt_trace_device", "pt_render", "pt_guide_create", "pt_guide_destroy", "pt_scene_set_guide",
+                 "pt_guide_get_weights", "pt_guide_set_weights", "pt_guide_eval", "pt_guide_pdf", "pt_guide_sample",
+                 "pt_guide_gradient", "pt_guide_step", "pt_guide_stats"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -223,12 +248,15 @@ class Scene:
 
     def __init__(self, ptgp):
         self._words = _words(ptgp)
+        self._guides = []
         h = ctypes.c_void_p()
         _check(library().pt_scene_create(self._words.ctypes.data, self._words.size, ctypes.byref(h)))
         self._h = h
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
+            for g in self._guides:   # a guide lives on its scene's stream: it goes first
+                g.close()
             _check(library().pt_scene_destroy(self._h))
             self._h = ctypes.c_void_p()
 
@@ -263,18 +291,25 @@ class Scene:
                                          ctypes.c_void_p(d_hits), ctypes.byref(ms), vis if count else None))
         return ms.value, (tuple(vis) if count else None)
 
+    def set_guide(self, guide: "Guide | None") -> None:
+        """Attaches a learned guide for guided / training renders (None detaches)."""
+        _check(library().pt_scene_set_guide(self._h, guide._h if guide is not None else None))
+
     def render(self, width: int, height: int, spp: int, max_bounces: int = 10, seed: int = 0,
                first_sample: int = 0, rows: tuple[int, int] | None = None, nee: bool = True, aux: bool = False,
                graph: bool = True, timing: bool = False, count: bool = False, samples_per_batch: int = 0,
-               stack_entries: int = 0, device_out: "DeviceBuffer | None" = None, tail: bool = True):
+               stack_entries: int = 0, device_out: "DeviceBuffer | None" = None, tail: bool = True,
+               guide: bool = False, train: bool = False):
         """Renders rows [j0, j1) (from the bottom; default all). Returns (rgb, aux, stats) with rgb of shape
-        (rows, width, 3), the band's top row first, as render_row_linear writes rows."""
+        (rows, width, 3), the band's top row first, as render_row_linear writes rows. guide / train: sample
+        (and train) the attached guide at diffuse bounces."""
         j0, j1 = rows if rows else (0, height)
         p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, j0, j1,
                           (0 if nee else RENDER_NO_NEE) | (0 if graph else RENDER_NO_GRAPH) |
                           (RENDER_TIMING if timing else 0) | (RENDER_COUNT if count else 0) |
                           (RENDER_DEVICE_OUTPUT if device_out is not None else 0) |
-                          (0 if tail else RENDER_NO_TAIL), samples_per_batch,
+                          (0 if tail else RENDER_NO_TAIL) | (RENDER_GUIDE if guide else 0) |
+                          (RENDER_TRAIN if train else 0), samples_per_batch,
                           stack_entries)
         nrows = j1 - j0
         st = _RenderStats()
@@ -292,6 +327,98 @@ class Scene:
         stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
                             tuple(st.visits), st.batches, st.samples_per_batch)
         return rgb, ax, stats
+
+
+def _f32(a, cols: int) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32).reshape(-1, cols)
+
+
+class Guide:
+    """A learned path guide on the GPU (pt_guide): an MLP giving an 8-lobe vMF mixture per shading point.
+    Training samples are rows of 12 floats: position, weight, normal, 0, direction, 0."""
+
+    def __init__(self, scene: Scene, seed: int = 0, learning_rate: float = 0.0, box=None,
+                 steps_per_batch: int = 0, minibatch: int = 0, weight_clamp: float = 0.0, bounces: int = 0):
+        p = _GuideParams()
+        p.seed = int(seed)
+        p.learning_rate = float(learning_rate)
+        if box is not None:
+            lo, hi = box
+            p.box_lo[:] = [float(v) for v in lo]
+            p.box_hi[:] = [float(v) for v in hi]
+        p.steps_per_batch = int(steps_per_batch)
+        p.minibatch = int(minibatch)
+        p.weight_clamp = float(weight_clamp)
+        p.bounces = int(bounces)
+        h = ctypes.c_void_p()
+        _check(library().pt_guide_create(scene._h, ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.scene = scene
+        scene._guides.append(self)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _check(library().pt_guide_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def weights(self) -> np.ndarray:
+        w = np.zeros(GUIDE_PARAMS, np.float32)
+        _check(library().pt_guide_get_weights(self._h, w.ctypes.data, w.size))
+        return w
+
+    def set_weights(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32).ravel()
+        _check(library().pt_guide_set_weights(self._h, w.ctypes.data, w.size))
+
+    def eval(self, pos, nrm) -> np.ndarray:
+        x, n = _f32(pos, 3), _f32(nrm, 3)
+        out = np.zeros((len(x), GUIDE_OUTPUTS), np.float32)
+        _check(library().pt_guide_eval(self._h, x.ctypes.data, n.ctypes.data, len(x), out.ctypes.data))
+        return out
+
+    def pdf(self, pos, nrm, dirs) -> np.ndarray:
+        x, n, d = _f32(pos, 3), _f32(nrm, 3), _f32(dirs, 3)
+        out = np.zeros(len(x), np.float32)
+        _check(library().pt_guide_pdf(self._h, x.ctypes.data, n.ctypes.data, d.ctypes.data, len(x), out.ctypes.data))
+        return out
+
+    def sample(self, pos, nrm, u) -> np.ndarray:
+        x, n, uu = _f32(pos, 3), _f32(nrm, 3), _f32(u, 3)
+        out = np.zeros((len(x), 4), np.float32)
+        _check(library().pt_guide_sample(self._h, x.ctypes.data, n.ctypes.data, uu.ctypes.data, len(x),
+                                         out.ctypes.data))
+        return out
+
+    def gradient(self, samples) -> tuple[np.ndarray, float]:
+        s = _f32(samples, 12)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = ctypes.c_float(0)
+        _check(library().pt_guide_gradient(self._h, s.ctypes.data, len(s), g.ctypes.data, ctypes.byref(loss)))
+        return g, loss.value
+
+    def step(self, samples) -> float:
+        s = _f32(samples, 12)
+        loss = ctypes.c_float(0)
+        _check(library().pt_guide_step(self._h, s.ctypes.data, len(s), ctypes.byref(loss)))
+        return loss.value
+
+    def stats(self) -> tuple[int, int, float]:
+        st, sa, lo = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_float(0)
+        _check(library().pt_guide_stats(self._h, ctypes.byref(st), ctypes.byref(sa), ctypes.byref(lo)))
+        return st.value, sa.value, lo.value
 
 
 class DeviceBuffer:

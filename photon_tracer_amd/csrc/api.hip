@@ -9,93 +9,11 @@
 #include <limits>
 #include <vector>
 
-#include "../../include/photon_tracer.h"
-#include "kernels.hpp"
-#include "status.hpp"
+#include "api_internal.hpp"
 
-using ptd::NONE;
-
-namespace {
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess)
-        throw pt::status_error(e == hipErrorOutOfMemory ? PT_ERR_NOMEM : PT_ERR_DEVICE,
-                               std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPC(x) hip_check((x), #x)
-
-struct dbuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void ensure(size_t n) {
-        if (n <= bytes && p) return;
-        release();
-        HIPC(hipMalloc(&p, n ? n : 16));
-        bytes = n;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-// Paths in flight per batch: up to 2^27 (a 1080p frame at 64 spp in one
-// batch), or what a quarter of the free HBM holds at kPathBytes each.
-constexpr size_t kMaxPaths = size_t(1) << 27;
-constexpr size_t kPathBytes = 2 * 36 + 24 + 48 + 48;
-
-struct render_bufs {
-    dbuf ro[2], rd[2], rpath[2], hits, so, sd, sl, pa, pb, pkey, pq, paux;
-    dbuf accum, aux_accum, ctrl, frame, totals, out_rgb, out_aux;
-    std::vector<dbuf*> all() {
-        return {&ro[0], &ro[1], &rd[0], &rd[1], &rpath[0], &rpath[1], &hits, &so, &sd, &sl, &pa, &pb, &pkey, &pq,
-                &paux, &accum, &aux_accum, &ctrl, &frame, &totals, &out_rgb, &out_aux};
-    }
-    void release() {
-        for (dbuf* b : all()) b->release();
-    }
-    size_t bytes() {
-        size_t n = 0;
-        for (dbuf* b : all()) n += b->bytes;
-        return n;
-    }
-};
-
-}  // namespace
-
-struct pt_scene {
-    int device = 0;
-    int cus = 0;
-    hipStream_t stream = nullptr;
-    std::vector<uint32_t> host;  // the PTGP words
-    dbuf words, leaf, spill, work, io_rays, io_hits, counts;
-    ptd::SceneView view{};
-    int max_stack = 0;   // entries a lane's stack can reach in this scene
-    int tuned_stack = 0, tuned_blocks = 0;
-    render_bufs rb;
-    // The last captured bounce-loop graph and the launch it was captured for.
-    struct graph_cache {
-        hipGraphExec_t exec = nullptr;
-        ptd::FrameParams key{};
-        int S = 0, trace_grid = 0, shade_grid = 0;
-    } graph;
-    void drop_graph() {
-        if (graph.exec) (void)hipGraphExecDestroy(graph.exec);
-        graph.exec = nullptr;
-    }
-    ~pt_scene() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        drop_graph();
-        rb.release();
-        for (dbuf* b : {&words, &leaf, &spill, &work, &io_rays, &io_hits, &counts}) b->release();
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+using ptapi::kMaxPaths;
+using ptapi::kPathBytes;
+using ptapi::render_bufs;
 
 namespace {
 
@@ -269,12 +187,12 @@ struct ptgp_check {
     }
 };
 
-int choose_spb(int spp, size_t npix, int requested, size_t free_bytes) {
+int choose_spb(int spp, size_t npix, int requested, size_t free_bytes, size_t extra_path_bytes) {
     if (requested > 0) {
         if (spp % requested) throw pt::status_error(PT_ERR_INVALID, "samples_per_batch must divide spp");
         return requested;
     }
-    const size_t target = std::min(kMaxPaths, free_bytes / 4 / kPathBytes);
+    const size_t target = std::min(kMaxPaths, free_bytes / 4 / (kPathBytes + extra_path_bytes));
     const size_t cap = std::max<size_t>(1, target / std::max<size_t>(1, npix));
     int best = 1;
     for (int d = 1; d <= spp; ++d)
@@ -472,7 +390,14 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         size_t free_bytes = 0, total_bytes = 0;
         HIPC(hipMemGetInfo(&free_bytes, &total_bytes));
         free_bytes += s->rb.bytes();  // buffers this scene already holds are reused
-        const int spb = choose_spb(P.spp, npix, P.samples_per_batch, free_bytes);
+        const bool training = (P.flags & PT_RENDER_TRAIN) != 0;
+        const bool guided = training || (P.flags & PT_RENDER_GUIDE) != 0;
+        pt_guide* G = s->guide;
+        if (guided && !G) throw pt::status_error(PT_ERR_INVALID, "pt_render: guided render without a guide (pt_scene_set_guide)");
+        // Per path: the guide's outputs; while training, its records, their
+        // weights and the compacted samples.
+        const size_t guide_bytes = (guided ? 64 : 0) + (training ? ptd::GUIDE_RECORDS * (64 + 4 + 48) : 0);
+        const int spb = choose_spb(P.spp, npix, P.samples_per_batch, free_bytes, guide_bytes);
         const size_t paths = npix * size_t(spb);
         if (paths >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render: batch too large");
         const int S = stack_entries_for(s, P.stack_entries);
@@ -544,6 +469,30 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         // Below four rays per resident traversal lane, a queue no longer fills
         // the GPU stage by stage and the tail launch takes the paths.
         fp.tail_threshold = (P.flags & PT_RENDER_NO_TAIL) ? 0u : static_cast<uint32_t>(trace_grid) * ptd::TRACE_BLOCK * 4u;
+        if (guided) {
+            fp.guide_mode = training ? 2 : 1;
+            fp.gv = G->view();
+            fp.gclamp = G->weight_clamp;
+            fp.guide_bounces = G->bounces;
+            B.gout.ensure(paths * ptd::guide::OUT * 2);
+            fp.gout = B.gout.as<_Float16>();
+            if (training) {
+                const size_t nrec = paths * ptd::GUIDE_RECORDS;
+                const size_t nblk = (nrec + ptd::GUIDE_TARGET_BLOCK - 1) / ptd::GUIDE_TARGET_BLOCK;
+                B.grec.ensure(nrec * 64);
+                B.gwt.ensure(nrec * 4);
+                B.gblk.ensure(nblk * 4);
+                B.gblk_sum.ensure(nblk * 4);
+                B.gtot.ensure(16);
+                B.gsamples.ensure(nrec * 48);
+                fp.grec = B.grec.as<float4>();
+                fp.gwt = B.gwt.as<float>();
+                fp.gblk = B.gblk.as<uint32_t>();
+                fp.gblk_sum = B.gblk_sum.as<float>();
+                fp.gtot = B.gtot.as<uint32_t>();
+                fp.gsamples = B.gsamples.as<float4>();
+            }
+        }
         const bool counting = stats && (P.flags & PT_RENDER_COUNT);
         if (counting) {
             s->counts.ensure(64);
@@ -575,6 +524,17 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
                 }
             }
             for (auto& e : ev) (void)hipEventDestroy(e);
+        } else if (training) {
+            // Eager: after each batch its records become samples, and the
+            // guide takes Adam steps over them before the next batch.
+            for (int b = 0; b < batches; ++b) {
+                HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr));
+                HIPC(ptd::launch_guide_targets(fp, s->stream));
+                uint32_t tot[2] = {0, 0};
+                HIPC(hipMemcpyAsync(tot, B.gtot.p, sizeof tot, hipMemcpyDeviceToHost, s->stream));
+                HIPC(hipStreamSynchronize(s->stream));
+                ptapi::guide_train_on(G, fp.gsamples, tot[0]);
+            }
         } else if ((P.flags & PT_RENDER_NO_GRAPH) || counting) {
             for (int b = 0; b < batches; ++b)
                 HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr));

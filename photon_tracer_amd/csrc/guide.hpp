@@ -1,0 +1,417 @@
+// The learned path guide on the device: input encoding, the MLP on FP16
+// MFMA tiles with FP32 accumulation, the 8-lobe von Mises-Fisher mixture it
+// outputs, and the gradient of the training loss.
+//
+// Network: one-blob position (3 x 16) + one-blob normal angles (2 x 8) = 64
+// inputs; four hidden layers of 64 with ReLU; 32 outputs = 8 lobes x
+// (theta logit, phi logit, log kappa, weight logit). Parameters are kept as
+// FP32 masters (Adam updates them) and mirrored in FP16, row-major [out][in]
+// and transposed [in][out], for the MFMA operands.
+//
+// A wave handles 64 samples. Every product is computed transposed, Y^T =
+// W X^T, on v_mfma_f32_32x32x16_f16: the A operand is a row of W (8 inputs a
+// lane), the B operand is a sample's row of X (8 inputs a lane), and the
+// 32x32 result leaves each lane one sample's column of 16 output features,
+// written back to LDS as four 8-byte runs of the next layer's row.
+#pragma once
+
+#include "device.hpp"
+
+namespace ptd {
+namespace guide {
+
+constexpr int LOBES = 8;
+constexpr int IN = 64, HID = 64, OUT = 4 * LOBES;
+constexpr int N_LAYERS = 5;
+constexpr int LAYER_STRIDE = HID * IN + HID;      // hidden layer: weights [64][64] then bias [64]
+constexpr int OUT_W = 4 * LAYER_STRIDE;           // output layer weights [32][64]
+constexpr int OUT_B = OUT_W + OUT * HID;
+constexpr int N_PARAMS = OUT_B + OUT;             // 18,720
+
+__host__ __device__ constexpr int w_off(int l) { return l < 4 ? l * LAYER_STRIDE : OUT_W; }
+__host__ __device__ constexpr int b_off(int l) { return l < 4 ? l * LAYER_STRIDE + HID * IN : OUT_B; }
+__host__ __device__ constexpr int n_out(int l) { return l < 4 ? HID : OUT; }
+
+// log kappa is clamped to [-3, 10]: kappa in [0.05, 22026], a lobe down to
+// about 0.4 degrees across (a 0.5 m opening seen from 10 m is 3 degrees).
+constexpr float LOG_KAPPA_MIN = -3.0f, LOG_KAPPA_MAX = 10.0f;
+
+// Row stride (halves) of a 64-wide activation tile in LDS: 144 bytes keeps
+// 16-byte alignment and spreads a wave's row reads over the banks.
+constexpr int XS = 72;
+constexpr int OS = 33;   // row stride (floats) of the FP32 output tile
+
+struct GuideView {
+    const _Float16* w16;    // FP16 weights, [out][in] at w_off(l)
+    const _Float16* wt16;   // FP16 weights transposed, [in][out] at w_off(l)
+    const float* p32;       // FP32 master parameters (biases are read from here)
+    float4 lo;              // position box: lower corner
+    float4 inv_ext;         // 1 / box extent per axis
+};
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+
+// One-blob encoding: a value in [0,1] spread over `bins` Gaussian bumps of
+// width one bin centred at (i + 0.5) / bins.
+__device__ __forceinline__ float blob(float v, int bins, int i) {
+    const float d = v * static_cast<float>(bins) - (static_cast<float>(i) + 0.5f);
+    return expf(-0.5f * d * d);
+}
+
+// The 64 inputs of a shading point: position normalised to the box, and the
+// normal's polar angle / pi and azimuth / (2 pi).
+__device__ __forceinline__ void encode(const GuideView& g, f3 x, f3 n, _Float16* row) {
+    const float c[3] = {clamp01((x.x - g.lo.x) * g.inv_ext.x), clamp01((x.y - g.lo.y) * g.inv_ext.y),
+                        clamp01((x.z - g.lo.z) * g.inv_ext.z)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) row[16 * a + i] = static_cast<_Float16>(blob(c[a], 16, i));
+    }
+    const float th = clamp01(acosf(fminf(fmaxf(n.z, -1.0f), 1.0f)) * (1 / PI_F));
+    const float ph = clamp01((atan2f(n.y, n.x) + PI_F) * (1 / (2 * PI_F)));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        row[48 + i] = static_cast<_Float16>(blob(th, 8, i));
+        row[56 + i] = static_cast<_Float16>(blob(ph, 8, i));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The mixture.
+// ---------------------------------------------------------------------------
+struct Lobe {
+    f3 mu;
+    float kappa, w;
+    float st, ct, sp, cp;   // sin/cos of the lobe's polar and azimuth angles
+    float sa, sb;           // sigmoid of the two angle logits
+    bool kappa_free;        // log kappa inside its clamp (its gradient passes)
+};
+
+__device__ __forceinline__ float sigmoid(float v) { return 1.0f / (1.0f + expf(-v)); }
+
+__device__ __forceinline__ void mixture(const float* o, Lobe* L) {
+    float mx = o[3];
+#pragma unroll
+    for (int k = 1; k < LOBES; ++k) mx = fmaxf(mx, o[4 * k + 3]);
+    float e[LOBES], sum = 0;
+#pragma unroll
+    for (int k = 0; k < LOBES; ++k) {
+        e[k] = expf(o[4 * k + 3] - mx);
+        sum += e[k];
+    }
+#pragma unroll
+    for (int k = 0; k < LOBES; ++k) {
+        Lobe& l = L[k];
+        l.w = e[k] / sum;
+        l.sa = sigmoid(o[4 * k]);
+        l.sb = sigmoid(o[4 * k + 1]);
+        const float th = PI_F * l.sa, ph = 2 * PI_F * l.sb;
+        l.st = sinf(th);
+        l.ct = cosf(th);
+        l.sp = sinf(ph);
+        l.cp = cosf(ph);
+        l.mu = mk(l.st * l.cp, l.st * l.sp, l.ct);
+        const float lk = o[4 * k + 2];
+        l.kappa_free = lk > LOG_KAPPA_MIN && lk < LOG_KAPPA_MAX;
+        l.kappa = expf(fminf(fmaxf(lk, LOG_KAPPA_MIN), LOG_KAPPA_MAX));
+    }
+}
+
+// vMF density on the sphere: kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}.
+__device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
+    return kappa / (2 * PI_F * -expm1f(-2 * kappa)) * expf(kappa * (dot(mu, w) - 1));
+}
+__device__ __forceinline__ float vmf_pdf(const Lobe& l, f3 w) { return vmf_pdf(l.mu, l.kappa, w); }
+
+// Lobe k's axis and concentration from the outputs o (FP32 or FP16).
+template <class T>
+__device__ __forceinline__ void lobe_at(const T* o, int k, f3& mu, float& kappa) {
+    const float th = PI_F * sigmoid(static_cast<float>(o[4 * k]));
+    const float ph = 2 * PI_F * sigmoid(static_cast<float>(o[4 * k + 1]));
+    const float st = sinf(th);
+    mu = mk(st * cosf(ph), st * sinf(ph), cosf(th));
+    kappa = expf(fminf(fmaxf(static_cast<float>(o[4 * k + 2]), LOG_KAPPA_MIN), LOG_KAPPA_MAX));
+}
+
+// The softmax of the weight logits as (max, sum of exp(logit - max)).
+template <class T>
+__device__ __forceinline__ void weight_norm(const T* o, float& mx, float& sum) {
+    mx = static_cast<float>(o[3]);
+#pragma unroll 1
+    for (int k = 1; k < LOBES; ++k) mx = fmaxf(mx, static_cast<float>(o[4 * k + 3]));
+    sum = 0;
+#pragma unroll 1
+    for (int k = 0; k < LOBES; ++k) sum += expf(static_cast<float>(o[4 * k + 3]) - mx);
+}
+
+// Mixture pdf at w, one lobe at a time (the loops stay rolled, so few
+// registers are live in the shade kernels, which run 16 waves a block).
+template <class T>
+__device__ __forceinline__ float mixture_pdf(const T* o, f3 w) {
+    float mx, sum;
+    weight_norm(o, mx, sum);
+    float p = 0;
+#pragma unroll 1
+    for (int k = 0; k < LOBES; ++k) {
+        f3 mu;
+        float kappa;
+        lobe_at(o, k, mu, kappa);
+        p += expf(static_cast<float>(o[4 * k + 3]) - mx) * vmf_pdf(mu, kappa, w);
+    }
+    return p / sum;
+}
+
+// A direction from the mixture: u0 picks the lobe by weight (the first whose
+// running weight passes u0, the last if rounding leaves u0 above the sum),
+// (u1, u2) sample it: the inversion c = 1 + log(u1 + (1 - u1) e^{-2 kappa}) /
+// kappa for the cosine to the lobe axis, a uniform azimuth.
+template <class T>
+__device__ __forceinline__ f3 mixture_sample(const T* o, float u0, float u1, float u2) {
+    float mx, sum;
+    weight_norm(o, mx, sum);
+    const float target = u0 * sum;
+    float acc = 0;
+    int sel = LOBES - 1;
+    bool chosen = false;
+#pragma unroll 1
+    for (int k = 0; k < LOBES; ++k) {
+        acc += expf(static_cast<float>(o[4 * k + 3]) - mx);
+        if (!chosen && target < acc) {
+            sel = k;
+            chosen = true;
+        }
+    }
+    f3 nn;
+    float kappa;
+    lobe_at(o, sel, nn, kappa);
+    const float cw = fminf(1.0f, fmaxf(-1.0f, 1 + logf(u1 + (1 - u1) * expf(-2 * kappa)) / kappa));
+    const float s = sqrtf(fmaxf(0.0f, 1 - cw * cw));
+    const float phi = 2 * PI_F * u2;
+    const float sign = copysignf(1.0f, nn.z);
+    const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
+    const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
+    const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
+    return t1 * (s * cosf(phi)) + t2 * (s * sinf(phi)) + nn * cw;
+}
+
+// Gradient of the sample's loss ws * -log p(w) with respect to the 32
+// outputs. Returns the loss.
+__device__ __forceinline__ float loss_grad(const float* o, f3 w, float ws, float* d) {
+    Lobe L[LOBES];
+    mixture(o, L);
+    float v[LOBES], p = 0;
+#pragma unroll
+    for (int k = 0; k < LOBES; ++k) {
+        v[k] = vmf_pdf(L[k], w);
+        p += L[k].w * v[k];
+    }
+    p = fmaxf(p, 1e-30f);
+#pragma unroll
+    for (int k = 0; k < LOBES; ++k) {
+        const Lobe& l = L[k];
+        const float r = l.w * v[k] / p;   // responsibility
+        // log v = log kappa - log 2pi - log(1 - e^{-2 kappa}) + kappa (mu.w - 1)
+        const float dlv_dk = 1 / l.kappa - 2 / expm1f(2 * l.kappa) + (dot(l.mu, w) - 1);
+        const f3 dmu_dth = mk(l.ct * l.cp, l.ct * l.sp, -l.st);
+        const f3 dmu_dph = mk(-l.st * l.sp, l.st * l.cp, 0.0f);
+        const float g = -ws * r;   // d loss / d log v
+        d[4 * k] = g * l.kappa * dot(w, dmu_dth) * (PI_F * l.sa * (1 - l.sa));
+        d[4 * k + 1] = g * l.kappa * dot(w, dmu_dph) * (2 * PI_F * l.sb * (1 - l.sb));
+        d[4 * k + 2] = l.kappa_free ? g * dlv_dk * l.kappa : 0.0f;
+        d[4 * k + 3] = ws * (l.w - r);
+    }
+    return -ws * logf(p);
+}
+
+// ---------------------------------------------------------------------------
+// MLP tiles. X, Y, dY: row-major [64 samples][XS] FP16 in LDS; XT, YT, dYT:
+// feature-major [features][XS] (sample i at column i). Lane = (c, kh):
+// c = lane & 31 picks the sample (or feature) within a 32-block, kh = lane >> 5
+// the half of each 16-wide k chunk. The two operands index k the same way,
+// so the sum runs over all k whatever order the unit takes them in.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_c() { return threadIdx.x & 31; }
+__device__ __forceinline__ int lane_kh() { return (threadIdx.x >> 5) & 1; }
+
+// Layer l forward: Y = act(X W^T + b). NOUT = 64 (hidden, ReLU) or 32 (the
+// output layer, no activation, FP32 into O). YT may be null.
+template <int NOUT>
+__device__ __forceinline__ void layer_forward(const GuideView& g, int l, const _Float16* X, _Float16* Y,
+                                              _Float16* YT, float* O) {
+    const int c = lane_c(), kh = lane_kh();
+    const _Float16* W = g.w16 + w_off(l);
+    const float* B = g.p32 + b_off(l);
+#pragma unroll
+    for (int mb = 0; mb < NOUT / 32; ++mb) {
+        h8 a[4];
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) a[kb] = *reinterpret_cast<const h8*>(W + (mb * 32 + c) * IN + kb * 16 + kh * 8);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            f16v acc;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = 0;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const h8 bx = *reinterpret_cast<const h8*>(X + (nb * 32 + c) * XS + kb * 16 + kh * 8);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bx, acc, 0, 0, 0);
+            }
+            const int i = nb * 32 + c;
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                const int m0 = mb * 32 + 8 * g4 + 4 * kh;
+                float v[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) v[t] = acc[4 * g4 + t] + B[m0 + t];
+                if (O) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) O[i * OS + m0 + t] = v[t];
+                } else {
+                    h4 hv;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) hv[t] = static_cast<_Float16>(fmaxf(v[t], 0.0f));
+                    *reinterpret_cast<h4*>(Y + i * XS + m0) = hv;
+                    if (YT) {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) YT[(m0 + t) * XS + i] = hv[t];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// The output layer of a wave's 64 rows X straight to HBM as FP16: row i (a
+// bit of `rows`) to out + i * OUT, four 8-byte runs per lane.
+__device__ __forceinline__ void output_store(const GuideView& g, const _Float16* X, unsigned long long rows,
+                                             _Float16* out) {
+    const int c = lane_c(), kh = lane_kh();
+    const _Float16* W = g.w16 + w_off(4);
+    const float* B = g.p32 + b_off(4);
+    h8 a[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) a[kb] = *reinterpret_cast<const h8*>(W + c * IN + kb * 16 + kh * 8);
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+        f16v acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0;
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            const h8 bx = *reinterpret_cast<const h8*>(X + (nb * 32 + c) * XS + kb * 16 + kh * 8);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bx, acc, 0, 0, 0);
+        }
+        const int i = nb * 32 + c;
+        if ((rows >> i) & 1ull) {
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                const int m0 = 8 * g4 + 4 * kh;
+                h4 hv;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) hv[t] = static_cast<_Float16>(acc[4 * g4 + t] + B[m0 + t]);
+                *reinterpret_cast<h4*>(out + i * OUT + m0) = hv;
+            }
+        }
+    }
+}
+
+// LDS of a forward-only wave: two ping-pong activation tiles and the outputs.
+struct EvalLds {
+    _Float16 a[64 * XS], b[64 * XS];
+    float o[64 * OS];
+};
+
+// The whole network on a wave's 64 encoded rows X[0]; hidden activations in
+// X[1..4] (and their transposes in XT[0..4] when XT is not null); outputs in O.
+__device__ __forceinline__ void forward(const GuideView& g, _Float16* const* X, _Float16* const* XT, float* O) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        layer_forward<HID>(g, l, X[l], X[l + 1], XT ? XT[l + 1] : nullptr, nullptr);
+        __syncthreads();
+    }
+    layer_forward<OUT>(g, 4, X[4], nullptr, nullptr, O);
+    __syncthreads();
+}
+
+// Layer l backward, given dY (= d loss / d layer output, row-major, NOUT
+// wide) and its transpose dYT: the FP32 weight gradient dW = dY^T X into
+// gw[j * 64 + k], the bias gradient into gb, and, when l > 0, the gradient
+// with respect to the layer's input masked by the ReLU that made it
+// (X > 0), into dX (row-major) and dXT.
+template <int NOUT>
+__device__ __forceinline__ void layer_backward(const GuideView& g, int l, const _Float16* dY, const _Float16* dYT,
+                                               const _Float16* X, const _Float16* XT, float* gw, float* gb,
+                                               _Float16* dX, _Float16* dXT, float unscale) {
+    const int c = lane_c(), kh = lane_kh();
+    // dW[j][k] = sum_i dY[i][j] X[i][k]: M = j, N = k, K = samples.
+#pragma unroll
+    for (int mb = 0; mb < NOUT / 32; ++mb) {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            f16v acc;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = 0;
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) {
+                const h8 a = *reinterpret_cast<const h8*>(dYT + (mb * 32 + c) * XS + kb * 16 + kh * 8);
+                const h8 bx = *reinterpret_cast<const h8*>(XT + (nb * 32 + c) * XS + kb * 16 + kh * 8);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bx, acc, 0, 0, 0);
+            }
+            const int k = nb * 32 + c;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = mb * 32 + 8 * (r / 4) + 4 * kh + (r % 4);
+                gw[j * IN + k] = acc[r] * unscale;
+            }
+        }
+    }
+    // Bias: sum over the samples, one feature per lane.
+    if (threadIdx.x < NOUT) {
+        float s = 0;
+        for (int i = 0; i < 64; ++i) s += static_cast<float>(dYT[threadIdx.x * XS + i]);
+        gb[threadIdx.x] = s * unscale;
+    }
+    if (l == 0) return;
+    // dX^T = W^T dY^T: M = input feature k, N = sample, K = output j.
+    const _Float16* WT = g.wt16 + w_off(l);
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        h8 a[NOUT / 16];
+#pragma unroll
+        for (int kb = 0; kb < NOUT / 16; ++kb)
+            a[kb] = *reinterpret_cast<const h8*>(WT + (mb * 32 + c) * NOUT + kb * 16 + kh * 8);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            f16v acc;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = 0;
+#pragma unroll
+            for (int kb = 0; kb < NOUT / 16; ++kb) {
+                const h8 bx = *reinterpret_cast<const h8*>(dY + (nb * 32 + c) * XS + kb * 16 + kh * 8);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bx, acc, 0, 0, 0);
+            }
+            const int i = nb * 32 + c;
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                const int m0 = mb * 32 + 8 * g4 + 4 * kh;
+                const h4 xm = *reinterpret_cast<const h4*>(X + i * XS + m0);
+                h4 hv;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    hv[t] = static_cast<float>(xm[t]) > 0.0f ? static_cast<_Float16>(acc[4 * g4 + t])
+                                                             : static_cast<_Float16>(0.0f);
+                *reinterpret_cast<h4*>(dX + i * XS + m0) = hv;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) dXT[(m0 + t) * XS + i] = hv[t];
+            }
+        }
+    }
+}
+
+}  // namespace guide
+}  // namespace ptd

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
+#include "guide.hpp"
 
 namespace ptd {
 
@@ -61,7 +62,23 @@ struct FrameParams {
     uint2* spill;             // traversal stack spill area
     unsigned long long* visits;  // null, or seven totals the traversal stage counts into (Visits)
     uint32_t tail_threshold;  // queues below this many rays go to the tail launch (0: never)
+    // The learned guide: 0 off, 1 diffuse bounces sample it (one-sample MIS
+    // with the cosine lobe), 2 they also record training vertices.
+    int guide_mode;
+    guide::GuideView gv;
+    _Float16* gout;           // per slot of the current bounce's queue: the 32 network outputs
+    float4* grec;             // GUIDE_RECORDS per path, 4 float4 each (layout at record_vertex)
+    float* gwt;               // per record: its training weight
+    uint32_t* gblk;           // per block of 256 records: count of weighted records, then its offset
+    float* gblk_sum;          // per block: sum of its weights
+    uint32_t* gtot;           // [0] training samples written, [1] float bits of their weight sum
+    float4* gsamples;         // compacted training samples, 3 float4 each
+    float gclamp;             // training weights are clamped at gclamp x their batch mean
+    int guide_bounces;        // diffuse vertices of bounces [0, guide_bounces) sample the guide
 };
+constexpr uint32_t GUIDE_RECORDS = 2;   // diffuse vertices per path recorded for training
+constexpr uint32_t GUIDE_TARGET_BLOCK = 256;
+
 
 // Traversal kernels for ray batches (pt_trace).
 // counts: null, or six 64-bit visit totals (Visits) to add the batch's into.
@@ -76,6 +93,9 @@ constexpr int TRACE_BLOCK = 256;
 hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
                         hipEvent_t* ev);
 hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStream_t stream);
+// After a training batch: its records' weights, compacted in path order into
+// fp.gsamples (count and weight sum in fp.gtot).
+hipError_t launch_guide_targets(const FrameParams& fp, hipStream_t stream);
 hipError_t launch_clear_frame(const FrameParams& fp, hipStream_t stream);
 
 }  // namespace ptd

@@ -123,6 +123,32 @@ __device__ __forceinline__ CameraSample camera_sample(const FrameParams& fp, con
     return cs;
 }
 
+// An unoccluded light sample of bounce bs: its radiance joins its path and,
+// while the guide trains, the record of the vertex that took it, whose
+// radiance then counts as arrived before the vertex's scattered ray.
+__device__ __forceinline__ void light_sample_arrives(const FrameParams& fp, uint32_t item, int bs) {
+    const float4 l = fp.sl[item];
+    const uint32_t p = __float_as_uint(l.w);
+    float4 r = fp.pb[p];
+    r.x = r.x + l.x;
+    r.y = r.y + l.y;
+    r.z = r.z + l.z;
+    fp.pb[p] = r;
+    if (fp.guide_mode == 2) {
+        const uint32_t k = (__float_as_uint(r.w) >> 1) & 3u;
+        if (k > 0) {
+            float4* R = fp.grec + (static_cast<size_t>(p) * GUIDE_RECORDS + k - 1) * 4;
+            if (__float_as_uint(R[0].w) == static_cast<uint32_t>(bs)) {
+                float4 t = R[3];
+                t.y = t.y + l.x;
+                t.z = t.z + l.y;
+                t.w = t.w + l.z;
+                R[3] = t;
+            }
+        }
+    }
+}
+
 // The traversal stage of bounce b: the light samples queued by bounce b - 1
 // (any hit, geometry::occluded: an unoccluded sample adds its radiance to its
 // path) and the rays of bounce b (closest hit, geometry::hit), drained by one
@@ -206,13 +232,7 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
             item &= 0x7fffffffu;
             if (any) {
                 if (w.kind == 0) {   // unoccluded: the light sample reaches its path
-                    const float4 l = fp.sl[item];
-                    const uint32_t p = __float_as_uint(l.w);
-                    float4 r = fp.pb[p];
-                    r.x = r.x + l.x;
-                    r.y = r.y + l.y;
-                    r.z = r.z + l.z;
-                    fp.pb[p] = r;
+                    light_sample_arrives(fp, item, b - 1);
                 }
             } else {
                 Hit h;
@@ -261,13 +281,7 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void
             if (!PRIMARY && item < n_shadow) {
                 const float4 a = fp.so[item], bb = fp.sd[item];
                 if (!trace_ray<true, COUNT, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st, &vis)) {
-                    const float4 l = fp.sl[item];
-                    const uint32_t p = __float_as_uint(l.w);
-                    float4 r = fp.pb[p];
-                    r.x = r.x + l.x;
-                    r.y = r.y + l.y;
-                    r.z = r.z + l.z;
-                    fp.pb[p] = r;
+                    light_sample_arrives(fp, item, b - 1);
                 }
                 continue;
             }
@@ -299,7 +313,18 @@ struct PathState {
     bool sampled;       // this bounce took a light sample (renderer.hpp:106-108,130)
     uint64_t key;       // the path's random stream
     uint2 pq;           // QMC pixel seed and sample index
+    uint32_t nrec;      // training records written (guide training)
 };
+
+// The path's flag word (pb[p].w): bit 0 `sampled`, bits 1-2 the records written.
+__device__ __forceinline__ float path_flags(const PathState& ps) {
+    return __uint_as_float((ps.sampled ? 1u : 0u) | (ps.nrec << 1));
+}
+__device__ __forceinline__ void read_flags(float w, PathState& ps) {
+    const uint32_t f = __float_as_uint(w);
+    ps.sampled = (f & 1u) != 0;
+    ps.nrec = (f >> 1) & 3u;
+}
 
 // What a bounce produced: the next ray (cont: from pnt along dir) and a light
 // sample to test (shadow: from pnt along sh_d up to sh_tmax, adding sh_l if
@@ -311,36 +336,21 @@ struct ShadeResult {
     float sh_tmax;
 };
 
-// ray_color's loop body for bounce b (renderer.hpp:109-146): the ray from o
-// along d, whose closest hit is h, of path p.
-__device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, uint32_t p, const Hit& h, f3 o, f3 d,
-                                                 PathState& ps) {
+// The hit record of a ray's closest hit (geometry.hpp:135-151,
+// sphere.hpp:50-58, triangle.hpp:63-74, geometry.hpp:231-244): point,
+// front-facing shading normal, material, and whether it is a light that
+// NEE samples.
+struct Surface {
+    f3 pnt, nrm;
+    bool front, sampled_light;
+    uint32_t mat;
+};
+
+__device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& h, f3 o, f3 d) {
     const SceneView& sc = fp.sc;
-    ShadeResult r;
-    r.cont = r.shadow = r.backface = false;
-    r.pnt = r.dir = r.sh_d = r.sh_l = mk(0, 0, 0);
-    r.sh_tmax = 0;
-    const bool was_sampled = ps.sampled;
-    ps.sampled = false;
-    if (h.kind == 0) {
-        // renderer.hpp:24-29
-        f3 bg = mk(0, 0, 0);
-        if (sc.flags & 1u) {
-            const f3 unit = normalize(d);
-            const float t = 0.5f * (unit.y + 1);
-            bg = mk(1, 1, 1) * (1 - t) + mk(0.5f, 0.7f, 1.0f) * t;
-        }
-        ps.rad = ps.rad + mul(ps.thr, bg);
-        if (fp.want_aux && b == 0) {
-            fp.paux[2 * p] = make_float4(1, 1, 1, 0);
-            fp.paux[2 * p + 1] = make_float4(0, 0, 0, 0);
-        }
-        return r;
-    }
-    // The hit record (geometry.hpp:135-151, sphere.hpp:50-58,
-    // triangle.hpp:63-74, geometry.hpp:231-244).
+    Surface sf;
     const f3 pnt = o + d * h.t;
-    r.pnt = pnt;
+    sf.pnt = pnt;
     f3 nrm;
     bool front;
     uint32_t mat;
@@ -378,6 +388,60 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
             if (im != NONE) mat = im;
         }
     }
+    sf.nrm = nrm;
+    sf.front = front;
+    sf.mat = mat;
+    sf.sampled_light = sampled_light;
+    return sf;
+}
+
+// Training record of a diffuse vertex (4 float4): (point, bounce),
+// (normal, dir.x), (dir.y, dir.z, thr.x, thr.y), (thr.z, base): thr is the
+// throughput before the vertex scatters, base the path's radiance once the
+// vertex's own emission and light sample are in, so that (final radiance -
+// base) / thr is what arrived along dir.
+__device__ __forceinline__ void record_vertex(const FrameParams& fp, uint32_t p, uint32_t k, int b, f3 pnt, f3 nrm,
+                                              f3 dir, f3 thr, f3 rad) {
+    float4* R = fp.grec + (static_cast<size_t>(p) * GUIDE_RECORDS + k) * 4;
+    R[0] = make_float4(pnt.x, pnt.y, pnt.z, __uint_as_float(static_cast<uint32_t>(b)));
+    R[1] = make_float4(nrm.x, nrm.y, nrm.z, dir.x);
+    R[2] = make_float4(dir.y, dir.z, thr.x, thr.y);
+    R[3] = make_float4(thr.z, rad.x, rad.y, rad.z);
+}
+
+// ray_color's loop body for bounce b (renderer.hpp:109-146): the ray from o
+// along d, whose closest hit is h, of path p. go: the guide's outputs for
+// this hit when diffuse bounces sample the guide, else null.
+template <bool GUIDED = false>
+__device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, uint32_t p, const Hit& h, f3 o, f3 d,
+                                                 PathState& ps, const _Float16* go) {
+    const SceneView& sc = fp.sc;
+    ShadeResult r;
+    r.cont = r.shadow = r.backface = false;
+    r.pnt = r.dir = r.sh_d = r.sh_l = mk(0, 0, 0);
+    r.sh_tmax = 0;
+    const bool was_sampled = ps.sampled;
+    ps.sampled = false;
+    if (h.kind == 0) {
+        // renderer.hpp:24-29
+        f3 bg = mk(0, 0, 0);
+        if (sc.flags & 1u) {
+            const f3 unit = normalize(d);
+            const float t = 0.5f * (unit.y + 1);
+            bg = mk(1, 1, 1) * (1 - t) + mk(0.5f, 0.7f, 1.0f) * t;
+        }
+        ps.rad = ps.rad + mul(ps.thr, bg);
+        if (fp.want_aux && b == 0) {
+            fp.paux[2 * p] = make_float4(1, 1, 1, 0);
+            fp.paux[2 * p + 1] = make_float4(0, 0, 0, 0);
+        }
+        return r;
+    }
+    const Surface sf = surface_at(fp, h, o, d);
+    const f3 pnt = sf.pnt, nrm = sf.nrm;
+    const bool front = sf.front, sampled_light = sf.sampled_light;
+    const uint32_t mat = sf.mat;
+    r.pnt = pnt;
     const float4 m0 = sc.mats[3 * mat], m1 = sc.mats[3 * mat + 1], m2 = sc.mats[3 * mat + 2];
     if (front && !(was_sampled && sampled_light)) ps.rad = ps.rad + mul(ps.thr, xyz(m1));
 
@@ -385,6 +449,8 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
     const int type = static_cast<int>(m0.w);
     f3 att = xyz(m0), dir = mk(0, 0, 0);
     bool ok = true;
+    f3 wgt = att;          // the scattered ray's throughput factor (att unless guided)
+    bool go_on = true;
     if (type == 0) {
         float a, bq;
         sample_2d(ps.pq.x, ps.pq.y, 1 + 2 * b, ps.key, ps.cnt, a, bq);
@@ -395,6 +461,26 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
         const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
         const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
         dir = t1 * (rr * cosf(phi)) + t2 * (rr * sinf(phi)) + nn * sqrtf(fmaxf(0.0f, 1 - a));
+        if (GUIDED && go) {
+            // One-sample MIS: the cosine lobe or the guide's mixture, 50/50,
+            // weighted by the mixed pdf, so the estimate stays unbiased for
+            // any guide. A direction below the surface carries nothing on.
+            if (rand01(ps.key, ps.cnt) >= 0.5f) {
+                const float u0 = rand01(ps.key, ps.cnt), u1 = rand01(ps.key, ps.cnt), u2 = rand01(ps.key, ps.cnt);
+                dir = guide::mixture_sample(go, u0, u1, u2);
+            }
+            const float cosv = dot(dir, nrm);
+            if (cosv > 0) {
+                const float pc = cosv * (1 / PI_F);
+                wgt = att * (pc / (0.5f * (pc + guide::mixture_pdf(go, dir))));
+                if (fp.guide_mode == 2 && ps.nrec < GUIDE_RECORDS) {
+                    record_vertex(fp, p, ps.nrec, b, pnt, nrm, dir, ps.thr, ps.rad);
+                    ++ps.nrec;
+                }
+            } else {
+                go_on = false;
+            }
+        }
     } else if (type == 1) {
         const f3 reflected = reflect(normalize(d), nrm);
         f3 ball = mk(0, 0, 0);
@@ -469,7 +555,9 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
         }
     }
     ps.sampled = nee;
-    ps.thr = mul(ps.thr, att);
+    if (!go_on) return r;
+    if (type != 0) wgt = att;
+    ps.thr = mul(ps.thr, wgt);
     if (b + 1 >= 3) {  // roulette_after (renderer.hpp:22,140-146)
         const float pm = fmaxf(ps.thr.x, fmaxf(ps.thr.y, ps.thr.z));
         if (pm < 1) {
@@ -484,10 +572,63 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
     return r;
 }
 
+// The guide's outputs for the shade stage of bounce b: for every queued hit
+// on a diffuse (Lambertian) surface, the network at its point and normal,
+// 32 FP16 values at gout[slot]. A block is one wave of 64 slots (18 KB of
+// LDS: two activation tiles; the output layer goes from the matrix unit's
+// registers to HBM); waves with no diffuse hit skip the network.
+template <bool PRIMARY>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_guide_infer(FrameParams fp, int b) {
+    if (fp.frame->tail_from <= static_cast<uint32_t>(b)) return;
+    __shared__ _Float16 xa[64 * guide::XS], xb[64 * guide::XS];
+    const uint32_t n = PRIMARY ? fp.P : fp.ctrl[b].n_rays;
+    FrameConst fc;
+    if (PRIMARY) fc = *fp.frame;
+    const int cur = b & 1;
+    for (uint32_t base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
+        const uint32_t q = base + threadIdx.x;
+        bool diffuse = false;
+        f3 pnt = mk(0, 0, 0), nrm = mk(0, 0, 1);
+        if (q < n) {
+            const Hit h = fp.hits[q];
+            if (h.kind != 0) {
+                f3 o, d;
+                if (PRIMARY) {
+                    const CameraSample cs = camera_sample(fp, fc, q);
+                    o = cs.o;
+                    d = cs.d;
+                } else {
+                    o = xyz(fp.ro[cur][q]);
+                    d = xyz(fp.rd[cur][q]);
+                }
+                const Surface sf = surface_at(fp, h, o, d);
+                diffuse = static_cast<int>(fp.sc.mats[3 * sf.mat].w) == 0;
+                pnt = sf.pnt;
+                nrm = sf.nrm;
+            }
+        }
+        const unsigned long long rows = __ballot(diffuse);
+        if (rows == 0) continue;   // the whole wave agrees
+        guide::encode(fp.gv, pnt, nrm, xa + threadIdx.x * guide::XS);
+        __syncthreads();
+        guide::layer_forward<guide::HID>(fp.gv, 0, xa, xb, nullptr, nullptr);
+        __syncthreads();
+        guide::layer_forward<guide::HID>(fp.gv, 1, xb, xa, nullptr, nullptr);
+        __syncthreads();
+        guide::layer_forward<guide::HID>(fp.gv, 2, xa, xb, nullptr, nullptr);
+        __syncthreads();
+        guide::layer_forward<guide::HID>(fp.gv, 3, xb, xa, nullptr, nullptr);
+        __syncthreads();
+        guide::output_store(fp.gv, xa, rows, fp.gout + static_cast<size_t>(base) * guide::OUT);
+        __syncthreads();   // the tiles are reused by the next slots
+    }
+}
+
 // The shade stage of bounce b over its queue. PRIMARY: bounce 0; the camera
 // ray and the fresh path state are rebuilt from the path number instead of
-// read back.
-template <bool PRIMARY>
+// read back. GUIDED: diffuse vertices sample the guide (its own kernel, so
+// the plain one carries none of that code's registers).
+template <bool PRIMARY, bool GUIDED = false>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     if (fp.frame->tail_from <= static_cast<uint32_t>(b)) return;  // the tail launch has taken these paths
     Ctrl* c = fp.ctrl + b;
@@ -517,6 +658,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 ps.rad = mk(0, 0, 0);
                 ps.cnt = 0;
                 ps.sampled = false;
+                ps.nrec = 0;
                 ps.key = path_key(fc.seed, cs.i, cs.j, cs.index);
                 ps.pq = make_uint2(cs.pseed, cs.index);
             } else {
@@ -529,14 +671,14 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 ps.thr = xyz(A);
                 ps.rad = xyz(B);
                 ps.cnt = __float_as_uint(A.w);
-                ps.sampled = (__float_as_uint(B.w) & 1u) != 0;
+                read_flags(B.w, ps);
                 o = xyz(o4);
                 d = xyz(d4);
             }
-            r = shade_hit(fp, b, p, h, o, d, ps);
+            r = shade_hit<GUIDED>(fp, b, p, h, o, d, ps, GUIDED ? fp.gout + static_cast<size_t>(q) * guide::OUT : nullptr);
             // The radiance always (accumulate reads it); the rest only for a
             // path that goes on.
-            fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, __uint_as_float(ps.sampled ? 1u : 0u));
+            fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, path_flags(ps));
             if (r.cont) {
                 fp.pa[p] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, __uint_as_float(ps.cnt));
                 if (PRIMARY) {
@@ -567,6 +709,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
 // stage, and no earlier bounce started it.
 __device__ __forceinline__ bool tail_starts(const FrameParams& fp, int b) {
     if (fp.frame->tail_from < static_cast<uint32_t>(b)) return false;
+    if (fp.guide_mode && b < fp.guide_bounces) return false;   // the tail shades without the guide
     return fp.ctrl[b - 1].n_shadow + fp.ctrl[b].n_rays < fp.tail_threshold;
 }
 
@@ -589,13 +732,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail_shadow(FrameParams fp, int
             const float4 a = fp.so[item], bb = fp.sd[item];
             Hit h;
             if (!trace_ray<true, false, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
-                const float4 l = fp.sl[item];
-                const uint32_t p = __float_as_uint(l.w);
-                float4 r = fp.pb[p];
-                r.x = r.x + l.x;
-                r.y = r.y + l.y;
-                r.z = r.z + l.z;
-                fp.pb[p] = r;
+                light_sample_arrives(fp, item, b - 1);
             }
         }
     }
@@ -632,14 +769,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
             ps.thr = xyz(A);
             ps.rad = xyz(B);
             ps.cnt = __float_as_uint(A.w);
-            ps.sampled = (__float_as_uint(B.w) & 1u) != 0;
+            read_flags(B.w, ps);
             f3 o = xyz(o4), d = xyz(d4);
             float tmin = o4.w, tmax = d4.w;
             for (int bb = b; bb < fp.depth; ++bb) {
                 Hit h;
                 trace_ray<false, false, INST>(fp.sc, o, d, tmin, tmax, h, st);
                 if (bb > b) ++rays;  // bounce b's rays were counted when they were queued
-                const ShadeResult r = shade_hit(fp, bb, p, h, o, d, ps);
+                const ShadeResult r = shade_hit(fp, bb, p, h, o, d, ps, nullptr);   // never with the guide
                 if (r.backface) ++backfaces;
                 if (r.shadow) {
                     ++shadows;
@@ -656,7 +793,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
                 tmin = T_MIN;
                 tmax = T_MAX;
             }
-            fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, 0.0f);
+            fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, path_flags(ps));
         }
     }
     // Per wave, one atomic per count.
@@ -815,13 +952,138 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
             else k_extend<S, false, INST, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
         }
         if (ev) (void)hipEventRecord(ev[2 * b + 1], stream);
-        if (b == 0) k_shade<true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
-        else k_shade<false><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        if (fp.guide_mode && b < fp.guide_bounces) {
+            const int infer_grid = (shade_grid / 2) * 8;   // 8 one-wave blocks per CU (18 KB of LDS each)
+            if (b == 0) k_guide_infer<true><<<infer_grid, 64, 0, stream>>>(fp, b);
+            else k_guide_infer<false><<<infer_grid, 64, 0, stream>>>(fp, b);
+        }
+        const bool guided = fp.guide_mode && b < fp.guide_bounces;
+        if (b == 0) {
+            if (guided) k_shade<true, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+            else k_shade<true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        } else {
+            if (guided) k_shade<false, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+            else k_shade<false><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        }
     }
     // No light samples are taken at the last bounce (renderer.hpp:127-130),
     // so no shadow rays are left over.
     k_accumulate<<<acc_grid, 256, 0, stream>>>(fp);
     k_finish<<<1, 64, 0, stream>>>(fp);
+}
+
+// Training weight of record k of path p, F/q = lum((final radiance - base)
+// / thr): the radiance that arrived along the sampled direction, times the
+// BSDF and cosine, over the pdf it was sampled with. Per block of 256
+// records: how many have weight, and their sum (fixed-order reductions).
+__global__ __launch_bounds__(GUIDE_TARGET_BLOCK) void k_guide_weights(FrameParams fp) {
+    __shared__ uint32_t wc[GUIDE_TARGET_BLOCK / 64];
+    __shared__ float wsum[GUIDE_TARGET_BLOCK / 64];
+    const uint32_t nrec = fp.P * GUIDE_RECORDS;
+    const uint32_t r = blockIdx.x * GUIDE_TARGET_BLOCK + threadIdx.x;
+    float w = 0;
+    if (r < nrec) {
+        const uint32_t p = r / GUIDE_RECORDS, k = r % GUIDE_RECORDS;
+        const float4 B = fp.pb[p];
+        if (k < ((__float_as_uint(B.w) >> 1) & 3u)) {
+            const float4 r2 = fp.grec[static_cast<size_t>(r) * 4 + 2], r3 = fp.grec[static_cast<size_t>(r) * 4 + 3];
+            const float tx = r2.z, ty = r2.w, tz = r3.x;
+            const float ax = fmaxf(B.x - r3.y, 0.0f), ay = fmaxf(B.y - r3.z, 0.0f), az = fmaxf(B.z - r3.w, 0.0f);
+            const float ex = tx > 0 ? ax / tx : 0.0f, ey = ty > 0 ? ay / ty : 0.0f, ez = tz > 0 ? az / tz : 0.0f;
+            w = 0.2126f * ex + 0.7152f * ey + 0.0722f * ez;
+            if (!(w > 0.0f) || !(w < 3.0e38f)) w = 0.0f;   // zero, NaN and infinity carry nothing
+        }
+        fp.gwt[r] = w;
+    }
+    const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(w > 0.0f)));
+    float sm = w;
+    for (int off = 32; off > 0; off >>= 1) sm += __shfl_xor(sm, off, 64);
+    if (lane_id() == 0) {
+        wc[threadIdx.x >> 6] = c;
+        wsum[threadIdx.x >> 6] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tc = 0;
+        float ts = 0;
+        for (int i = 0; i < GUIDE_TARGET_BLOCK / 64; ++i) {
+            tc += wc[i];
+            ts += wsum[i];
+        }
+        fp.gblk[blockIdx.x] = tc;
+        fp.gblk_sum[blockIdx.x] = ts;
+    }
+}
+
+// The blocks' counts become offsets (exclusive scan, one block, each thread
+// a contiguous run of blocks); gtot gets the count and the weight sum.
+__global__ __launch_bounds__(1024) void k_guide_scan(FrameParams fp, uint32_t nblk) {
+    __shared__ uint32_t part[1024];
+    __shared__ float psum[1024];
+    const uint32_t chunk = (nblk + 1023) / 1024;
+    const uint32_t lo = min(threadIdx.x * chunk, nblk), hi = min(lo + chunk, nblk);
+    uint32_t c = 0;
+    float sm = 0;
+    for (uint32_t i = lo; i < hi; ++i) {
+        c += fp.gblk[i];
+        sm += fp.gblk_sum[i];
+    }
+    part[threadIdx.x] = c;
+    psum[threadIdx.x] = sm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        float total = 0;
+        for (int t = 0; t < 1024; ++t) {
+            const uint32_t v = part[t];
+            part[t] = run;
+            run += v;
+            total += psum[t];
+        }
+        fp.gtot[0] = run;
+        fp.gtot[1] = __float_as_uint(total);
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = fp.gblk[i];
+        fp.gblk[i] = run;
+        run += v;
+    }
+}
+
+// The weighted records, in record order, as training samples: (point,
+// weight / batch mean, clamped), (normal, 0), (direction, 0).
+__global__ __launch_bounds__(GUIDE_TARGET_BLOCK) void k_guide_compact(FrameParams fp) {
+    __shared__ uint32_t wc[GUIDE_TARGET_BLOCK / 64];
+    const uint32_t nrec = fp.P * GUIDE_RECORDS;
+    const uint32_t r = blockIdx.x * GUIDE_TARGET_BLOCK + threadIdx.x;
+    const float w = r < nrec ? fp.gwt[r] : 0.0f;
+    const unsigned long long m = __ballot(w > 0.0f);
+    if (lane_id() == 0) wc[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    if (!(w > 0.0f)) return;
+    uint32_t slot = fp.gblk[blockIdx.x];
+    for (uint32_t i = 0; i < (threadIdx.x >> 6); ++i) slot += wc[i];
+    slot += static_cast<uint32_t>(__popcll(m & ((1ull << lane_id()) - 1ull)));
+    const float total = __uint_as_float(fp.gtot[1]);
+    const float mean = total / static_cast<float>(fp.gtot[0]);
+    const float ws = fminf(w / mean, fp.gclamp);
+    const float4* R = fp.grec + static_cast<size_t>(r) * 4;
+    const float4 r0 = R[0], r1 = R[1], r2 = R[2];
+    float4* S = fp.gsamples + static_cast<size_t>(slot) * 3;
+    S[0] = make_float4(r0.x, r0.y, r0.z, ws);
+    S[1] = make_float4(r1.x, r1.y, r1.z, 0.0f);
+    S[2] = make_float4(r1.w, r2.x, r2.y, 0.0f);
+}
+
+hipError_t launch_guide_targets(const FrameParams& fp, hipStream_t stream) {
+    const uint32_t nrec = fp.P * GUIDE_RECORDS;
+    const uint32_t nblk = (nrec + GUIDE_TARGET_BLOCK - 1) / GUIDE_TARGET_BLOCK;
+    k_guide_weights<<<nblk, GUIDE_TARGET_BLOCK, 0, stream>>>(fp);
+    k_guide_scan<<<1, 1024, 0, stream>>>(fp, nblk);
+    k_guide_compact<<<nblk, GUIDE_TARGET_BLOCK, 0, stream>>>(fp);
+    return hipGetLastError();
 }
 
 hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,

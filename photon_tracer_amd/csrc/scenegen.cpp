@@ -399,14 +399,17 @@ scene_desc guided_room(uint64_t seed, double scale) {
     add_quad(s, v3(px, hy1, z0), v3(0, 0, z1 - z0), v3(0, y1 - hy1, 0), white);    // above
     add_quad(s, v3(px, hy0, z0), v3(0, 0, hz0 - z0), v3(0, hy1 - hy0, 0), white);  // beside, -z
     add_quad(s, v3(px, hy0, hz1), v3(0, 0, z1 - hz1), v3(0, hy1 - hy0, 0), white); // beside, +z
+    // The mesh is diffuse (two albedos by face hash): the scene tests how
+    // diffuse interreflection finds light through the opening, the transport
+    // a guide for diffuse bounces addresses.
     const uint32_t matte = s.add_material(lambertian(0.65, 0.6, 0.55));
-    const uint32_t shiny = s.add_material(metal(0.9, 0.8, 0.6, 0.2));
+    const uint32_t matte2 = s.add_material(lambertian(0.45, 0.5, 0.6));
     const int n = std::max(2, static_cast<int>(std::lround(224 * std::sqrt(scale))));
     indexed_mesh m = geodesic_sphere(n);
     const value_noise noise{hash32(static_cast<uint32_t>(seed) ^ 0x7001u)};
     const uint32_t fs = hash32(static_cast<uint32_t>(seed >> 32) ^ 0x7002u);
     auto mesh = finish_mesh(m, noise, 4.0, 0.08, 1.2, v3(-1.0f, 1.25f, 0.0f), [&](uint32_t f) {
-        return (hash32(f ^ fs) % 10u) < 7u ? matte : shiny;
+        return (hash32(f ^ fs) % 10u) < 7u ? matte : matte2;
     });
     s.triangles.insert(s.triangles.end(), mesh.begin(), mesh.end());
     const uint32_t lamp = s.add_material(light(2000));
