@@ -198,6 +198,89 @@ pt_status pt_device_free(void* ptr);
 pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int kind);
 
 /* ---------------------------------------------------------------------------
+ * Batched rendering (the config-3 workload): many jobs -- scenes, views, image
+ * sizes and spp budgets of uneven size -- in one call. The jobs' scenes are
+ * merged into one device scene; each job runs pt_render's bounce loop over
+ * its view of it, the jobs spread over a few HIP streams that run side by
+ * side, each stream's whole job sequence one captured graph. The reference
+ * renders one image per call (render_row_linear, renderer.hpp:411-416); a
+ * caller with many jobs calls it once per job. Every job's image is bit for
+ * bit what pt_render gives for the same scene, size, spp, seed and
+ * first_sample.
+ * ------------------------------------------------------------------------- */
+typedef struct pt_batch pt_batch;
+
+/* Where one scene of a batch sits in the merged scene (pt_batch_merge). */
+typedef struct {
+    uint32_t root;              /* flat tree root node, or 0xffffffff (no tree) */
+    uint32_t n_large, large_off;/* its large spheres: merged large-sphere list [large_off, +n_large) */
+    uint32_t n_lights, lights_off;
+    uint32_t flags;             /* its PTGP header flags (1 sky, 2 sample lights) */
+    float camera[16];           /* its PTGP camera rows: origin, corner, across, up (w = 0) */
+} pt_batch_part;
+
+/* Host only: several flat PTGP v1 scenes (no instances) as one PTGP v1
+ * buffer (free with pt_free): arrays concatenated with rebased indices, and
+ * 4-wide nodes over the scenes' roots as its flat root. parts: n entries. */
+pt_status pt_batch_merge(const uint32_t* const* ptgps, const size_t* n_words, int n, uint32_t** merged,
+                         size_t* merged_words, pt_batch_part* parts);
+
+/* Uploads the merged scene of n flat PTGP v1 scenes (each checked as
+ * pt_scene_create checks one). */
+pt_status pt_batch_create(const uint32_t* const* ptgps, const size_t* n_words, int n, pt_batch** batch);
+pt_status pt_batch_destroy(pt_batch* batch);
+
+enum { PT_JOB_CAMERA = 0x100 };   /* pt_batch_job.flags: use the job's camera rows, not its scene's */
+
+typedef struct {
+    int32_t scene;              /* which of the batch's scenes */
+    int32_t width, height;      /* image size, 2 .. 32768 */
+    int32_t spp;                /* samples per pixel */
+    uint64_t seed;
+    int32_t first_sample;       /* progressive continuation, as pt_render_params */
+    int32_t flags;              /* PT_RENDER_NO_NEE, PT_JOB_CAMERA */
+    float camera[16];           /* with PT_JOB_CAMERA: origin, corner, across, up (4 floats each, w unused) */
+} pt_batch_job;
+
+typedef struct {
+    int32_t max_bounces;        /* every job's bounce cap, 1 .. 63 */
+    int32_t flags;              /* PT_RENDER_NO_GRAPH, PT_RENDER_DEVICE_OUTPUT, PT_RENDER_NO_TAIL */
+    int32_t stack_entries;      /* 8, 16 or 32; 0 = tuned (pt_batch_autotune) or 8 */
+    int32_t blocks_per_cu;      /* traversal blocks per CU; 0 = tuned or the occupancy limit */
+    int32_t streams;            /* HIP streams the jobs are spread over, 1 .. 8; 0 = tuned or 2 */
+    int32_t reserved;
+    uint64_t max_paths;         /* paths one batch of a job may hold (the queue size); 0 = tuned or memory-sized */
+} pt_batch_params;
+
+/* Renders every job. rgb: the jobs' images one after another in job order,
+ * each 3 floats a pixel, top row first (a host array, or with
+ * PT_RENDER_DEVICE_OUTPUT a device one). A job's samples go in batches of
+ * spb samples a pixel, spb the largest divisor of its spp whose batch fits
+ * max_paths; a pixel's samples are summed in sample order, so the batch size
+ * changes no bit of the result. stats: rays as pt_render counts them, summed
+ * over the jobs; batches: summed over the jobs; render_ms: first stream start
+ * to last stream end. */
+pt_status pt_render_batch(pt_batch* batch, const pt_batch_job* jobs, int n_jobs, const pt_batch_params* params,
+                          float* rgb, pt_render_stats* stats);
+
+/* The autotuner: renders the jobs at a sixteenth of their spp under candidate
+ * launch shapes -- LDS stack entries (8, 16, 32) x traversal blocks per CU,
+ * then streams (1, 2, 4), then batch sizes (2^22, 2^24 paths) -- and keeps the
+ * fastest in the batch for later renders whose params leave them 0. result
+ * (optional): the chosen shape and the measured rate of each candidate. */
+typedef struct {
+    int32_t stack_entries, blocks_per_cu, streams, n_tried;
+    uint64_t max_paths;
+    struct {
+        int32_t stack_entries, blocks_per_cu, streams, reserved;
+        uint64_t max_paths;
+        double mrays_per_s;
+    } tried[32];
+} pt_tune_result;
+pt_status pt_batch_autotune(pt_batch* batch, const pt_batch_job* jobs, int n_jobs, int max_bounces,
+                            pt_tune_result* result);
+
+/* ---------------------------------------------------------------------------
  * Learned path guide (the config-4 guided tracer). The reference has no path
  * guiding: these entry points have no counterpart there. The guide is an MLP
  * (one-blob position 3x16 + normal angles 2x8 = 64 inputs, four hidden layers

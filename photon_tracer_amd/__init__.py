@@ -13,6 +13,8 @@ Reference correspondence (apollo-2006/photon_tracer):
   Scene.trace       geometry::hit / geometry::occluded (geometry.hpp:111-190)
   Scene.render      render_row_linear over a frame or band of rows
                     (renderer.hpp:366-416)
+  Batch.render      many render_row_linear calls (jobs of several scenes,
+                    views, sizes and spp) as one launch sequence
 """
 from __future__ import annotations
 
@@ -39,6 +41,7 @@ RENDER_GUIDE = 64
 RENDER_TRAIN = 128
 GUIDE_PARAMS = 18720
 GUIDE_OUTPUTS = 32
+JOB_CAMERA = 0x100
 
 # Algorithmic bytes of one traversal query (the ray in, the hit out, and every
 # record the reference's walk must read): 32 + 16 per query, 128 per node slab
@@ -77,6 +80,34 @@ class _GuideParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("learning_rate", ctypes.c_float), ("box_lo", ctypes.c_float * 3),
                 ("box_hi", ctypes.c_float * 3), ("steps_per_batch", ctypes.c_int32), ("minibatch", ctypes.c_int32),
                 ("weight_clamp", ctypes.c_float), ("bounces", ctypes.c_int32)]
+
+
+class _BatchPart(ctypes.Structure):
+    _fields_ = [("root", ctypes.c_uint32), ("n_large", ctypes.c_uint32), ("large_off", ctypes.c_uint32),
+                ("n_lights", ctypes.c_uint32), ("lights_off", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("camera", ctypes.c_float * 16)]
+
+
+class _BatchJob(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("spp", ctypes.c_int32), ("seed", ctypes.c_uint64), ("first_sample", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("camera", ctypes.c_float * 16)]
+
+
+class _BatchParams(ctypes.Structure):
+    _fields_ = [("max_bounces", ctypes.c_int32), ("flags", ctypes.c_int32), ("stack_entries", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("streams", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("max_paths", ctypes.c_uint64)]
+
+
+class _Tried(ctypes.Structure):
+    _fields_ = [("stack_entries", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("streams", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("max_paths", ctypes.c_uint64), ("mrays_per_s", ctypes.c_double)]
+
+
+class _TuneResult(ctypes.Structure):
+    _fields_ = [("stack_entries", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32), ("streams", ctypes.c_int32),
+                ("n_tried", ctypes.c_int32), ("max_paths", ctypes.c_uint64), ("tried", _Tried * 32)]
 
 
 class _JobDesc(ctypes.Structure):
@@ -131,6 +162,14 @@ def library() -> ctypes.CDLL:
     lib.pt_guide_gradient.argtypes = [vp, vp, sz, vp, fp]
     lib.pt_guide_step.argtypes = [vp, vp, sz, fp]
     lib.pt_guide_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), fp]
+    lib.pt_batch_merge.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz), vp]
+    lib.pt_batch_create.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.pt_batch_destroy.argtypes = [vp]
+    lib.pt_render_batch.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(_BatchParams), vp,
+                                    ctypes.POINTER(_RenderStats)]
+    lib.pt_batch_autotune.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_TuneResult)]
+    for name in ("pt_batch_merge", "pt_batch_create", "pt_batch_destroy", "pt_render_batch", "pt_batch_autotune"):
+        getattr(lib, name).restype = ctypes.c_int
     for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate",
                  "pt_generate_job", "pt_pack", "pt_ptgp_depth", "pt_scene_create", "pt_scene_destroy", "pt_trace",
                  "pt_trace_device", "pt_render", "pt_guide_create", "pt_guide_destroy", "pt_scene_set_guide",
@@ -468,3 +507,132 @@ def read_pfm(path: str) -> np.ndarray:
         scale = float(f.readline())
         data = np.frombuffer(f.read(), dtype="<f4" if scale < 0 else ">f4")
     return data.reshape(h, w, 3)[::-1].astype(np.float32)
+
+
+# ---- batched renders ------------------------------------------------------------
+
+@dataclass
+class BatchJob:
+    """One job of a batched render: scene number in the batch, image size, spp, seed, and optionally its
+    own camera rows (4 x 4 floats: origin, corner, across, up) instead of its scene's."""
+    scene: int
+    width: int
+    height: int
+    spp: int
+    seed: int = 0
+    first_sample: int = 0
+    nee: bool = True
+    camera: "np.ndarray | None" = None
+
+
+@dataclass
+class BatchPart:
+    root: int
+    n_large: int
+    large_off: int
+    n_lights: int
+    lights_off: int
+    flags: int
+    camera: np.ndarray
+
+
+def _scene_arrays(ptgps):
+    ws = [_words(p) for p in ptgps]
+    ptrs = (ctypes.c_void_p * len(ws))(*[w.ctypes.data for w in ws])
+    sizes = (ctypes.c_size_t * len(ws))(*[w.size for w in ws])
+    return ws, ptrs, sizes
+
+
+def batch_merge(ptgps) -> tuple[np.ndarray, list[BatchPart]]:
+    """Host only: several flat PTGP scenes merged into one PTGP buffer, and where each scene went."""
+    ws, ptrs, sizes = _scene_arrays(ptgps)
+    parts = (_BatchPart * len(ws))()
+    p, n = _u32p(), ctypes.c_size_t(0)
+    _check(library().pt_batch_merge(ptrs, sizes, len(ws), ctypes.byref(p), ctypes.byref(n), parts))
+    out = [BatchPart(q.root, q.n_large, q.large_off, q.n_lights, q.lights_off, q.flags,
+                     np.array(q.camera[:], np.float32).reshape(4, 4)) for q in parts]
+    return _take_words(p, n), out
+
+
+def _job_array(jobs):
+    arr = (_BatchJob * len(jobs))()
+    for k, j in enumerate(jobs):
+        a = arr[k]
+        a.scene, a.width, a.height, a.spp = int(j.scene), int(j.width), int(j.height), int(j.spp)
+        a.seed, a.first_sample = int(j.seed), int(j.first_sample)
+        a.flags = (0 if j.nee else RENDER_NO_NEE) | (JOB_CAMERA if j.camera is not None else 0)
+        if j.camera is not None:
+            a.camera[:] = [float(v) for v in np.asarray(j.camera, np.float32).ravel()[:16]]
+    return arr
+
+
+@dataclass
+class TuneResult:
+    stack_entries: int
+    blocks_per_cu: int
+    streams: int
+    max_paths: int
+    tried: list
+
+
+class Batch:
+    """Several flat scenes merged on the GPU (pt_batch), rendered many jobs at a time."""
+
+    def __init__(self, ptgps):
+        self._ws, ptrs, sizes = _scene_arrays(ptgps)
+        h = ctypes.c_void_p()
+        _check(library().pt_batch_create(ptrs, sizes, len(self._ws), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _check(library().pt_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, jobs, max_bounces: int = 10, graph: bool = True, tail: bool = True, stack_entries: int = 0,
+               blocks_per_cu: int = 0, streams: int = 0, max_paths: int = 0, device_out: "DeviceBuffer | None" = None):
+        """Renders every job. Returns (images, stats): one (height, width, 3) array a job, top row first
+        (None with device_out: the images stay in that device buffer, one after another)."""
+        arr = _job_array(jobs)
+        flags = (0 if graph else RENDER_NO_GRAPH) | (0 if tail else RENDER_NO_TAIL) | \
+            (RENDER_DEVICE_OUTPUT if device_out is not None else 0)
+        p = _BatchParams(int(max_bounces), flags, int(stack_entries), int(blocks_per_cu), int(streams), 0,
+                         int(max_paths))
+        sizes = [int(j.width) * int(j.height) for j in jobs]
+        st = _RenderStats()
+        if device_out is not None:
+            assert device_out.nbytes >= 12 * sum(sizes)
+            _check(library().pt_render_batch(self._h, arr, len(jobs), ctypes.byref(p), ctypes.c_void_p(device_out.ptr),
+                                             ctypes.byref(st)))
+            stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
+                                tuple(st.visits), st.batches, st.samples_per_batch)
+            return None, stats
+        rgb = np.zeros(3 * sum(sizes), np.float32)
+        _check(library().pt_render_batch(self._h, arr, len(jobs), ctypes.byref(p), rgb.ctypes.data, ctypes.byref(st)))
+        out, at = [], 0
+        for j, n in zip(jobs, sizes):
+            out.append(rgb[3 * at:3 * (at + n)].reshape(int(j.height), int(j.width), 3))
+            at += n
+        stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
+                            tuple(st.visits), st.batches, st.samples_per_batch)
+        return out, stats
+
+    def autotune(self, jobs, max_bounces: int = 10) -> TuneResult:
+        """Times short renders of the jobs under candidate launch shapes and keeps the fastest for this batch."""
+        arr = _job_array(jobs)
+        r = _TuneResult()
+        _check(library().pt_batch_autotune(self._h, arr, len(jobs), int(max_bounces), ctypes.byref(r)))
+        tried = [(t.stack_entries, t.blocks_per_cu, t.streams, t.max_paths, t.mrays_per_s) for t in r.tried[:r.n_tried]]
+        return TuneResult(r.stack_entries, r.blocks_per_cu, r.streams, r.max_paths, tried)

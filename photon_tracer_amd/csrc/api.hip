@@ -200,6 +200,15 @@ int choose_spb(int spp, size_t npix, int requested, size_t free_bytes, size_t ex
     return best;
 }
 
+}  // namespace
+
+namespace ptapi {
+
+void check_ptgp(const uint32_t* w, size_t n) {
+    ptgp_check chk{w, n};
+    chk.run();
+}
+
 int stack_entries_for(pt_scene* s, int requested) {
     // Without instances the 8-entry ring runs at 8 waves per SIMD; with them
     // two trees are live at once and 16 entries spill less.
@@ -219,7 +228,11 @@ void ensure_spill(pt_scene* s, int grid) {
     s->spill.ensure(lanes * size_t(s->max_stack) * sizeof(uint2));
 }
 
-}  // namespace
+}  // namespace ptapi
+
+using ptapi::ensure_spill;
+using ptapi::stack_entries_for;
+using ptapi::trace_grid_for;
 
 extern "C" {
 

@@ -140,4 +140,14 @@ namespace ptapi {
 // Adam steps of a guide over n device samples (3 float4 each), in
 // minibatches of g->minibatch, steps_per_batch of them (0: one pass).
 void guide_train_on(pt_guide* g, const float4* samples, uint32_t n);
+
+// Checks a PTGP v1 buffer as pt_scene_create does (throws status_error).
+void check_ptgp(const uint32_t* w, size_t n);
+// LDS stack entries for a render (8, 16 or 32): requested, else tuned, else
+// the default for the scene.
+int stack_entries_for(pt_scene* s, int requested);
+// Traversal grid: blocks per CU (tuned, else the occupancy limit) x CUs.
+int trace_grid_for(pt_scene* s, int S);
+// The traversal stacks' spill area for a grid of that many blocks.
+void ensure_spill(pt_scene* s, int grid);
 }  // namespace ptapi

@@ -171,6 +171,21 @@ box3 triangle_box(const triangle_rec& t);
 // PTSD -> PTGP v1 words (geometry::build + pack_for_gpu + serialize).
 std::vector<uint32_t> pack_ptgp(const scene_desc& s);
 
+// Batched renders (merge.cpp): several flat PTGP v1 scenes (no instances) as
+// one, every array concatenated with its indices rebased and a few 4-wide
+// nodes over the scenes' roots as the merged flat root, so the result is an
+// ordinary PTGP v1 buffer. parts: where each scene's tree root, large spheres
+// and lights went, with its header flags and camera rows.
+struct ptgp_in {
+    const uint32_t* w;
+    size_t n;
+};
+struct merge_part {
+    uint32_t root, n_large, large_off, n_lights, lights_off, flags;
+    float camera[16];
+};
+std::vector<uint32_t> merge_ptgp(const std::vector<ptgp_in>& in, std::vector<merge_part>& parts);
+
 // Scene generators for the benchmark configurations (scenegen.cpp).
 scene_desc generate_config(int config, uint64_t seed, double scale);
 struct job_desc {
