@@ -21,6 +21,17 @@ Besides the timed steps it reports:
   cpu_baseline  the reference's own renderer (oracle/_ref, its headers
                 compiled with its flags) on the host cores, on a bounded
                 sample of the same frame.
+  batch         the batched workload: 64 seeded jobs (sphere fields of
+                50-2000 spheres or displaced meshes of 10k-1M triangles;
+                256^2 to 1024^2; 4, 16 or 64 spp) in one pt_render_batch call
+                after the autotuner, against the same jobs rendered one
+                pt_render call at a time.
+  guided        the hard-lighting room (a 1M-triangle displaced mesh, lamps
+                behind a partition with a 0.5 x 0.5 opening) at 1920x1080:
+                relMSE against a 4096-spp plain render, plain NEE against the
+                learned guide at equal device time (a quarter of it spent
+                training), and the time the guided tracer needs to reach the
+                plain tracer's relMSE.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -98,6 +109,132 @@ def cpu_baseline(ptsd, width, height, spp_full, seconds):
     }
 
 
+def batch_leg(pt, n_jobs, steps):
+    """The batched workload: n_jobs seeded jobs, generated and packed on host
+    threads, merged into one device scene, tuned, then rendered in one call
+    per step; the same jobs one pt_render call at a time for comparison."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        jobs = list(ex.map(lambda s: pt.generate_job(s, 1.0), range(n_jobs)))
+        ptgps = list(ex.map(lambda j: pt.pack(j.ptsd), jobs))
+    t1 = time.perf_counter()
+    n_pix = sum(j.width * j.height for j in jobs)
+    n_paths = sum(j.width * j.height * j.spp for j in jobs)
+    n_tris = sum(j.primitives for j in jobs if j.kind == 1)
+    log(f"batch: {n_jobs} jobs ({sum(j.kind == 1 for j in jobs)} meshes, {n_tris} triangles), {n_pix} pixels, "
+        f"{n_paths} paths; generated and packed in {t1 - t0:.1f} s")
+    bjobs = [pt.BatchJob(k, j.width, j.height, j.spp, seed=j.seed) for k, j in enumerate(jobs)]
+    out = {}
+    with pt.Batch(ptgps) as b:
+        t2 = time.perf_counter()
+        tune = b.autotune(bjobs)
+        log(f"batch: autotuned in {time.perf_counter() - t2:.1f} s: stack {tune.stack_entries}, "
+            f"blocks/CU {tune.blocks_per_cu}, streams {tune.streams}, paths {tune.max_paths}")
+        out_buf = pt.DeviceBuffer(12 * n_pix)   # the images stay in HBM, as in the timed frames above
+        _, st = b.render(bjobs, device_out=out_buf)   # warmup (captures the graphs)
+        rays, ms, walls = 0, 0.0, []
+        for k in range(steps):
+            t3 = time.perf_counter()
+            _, st = b.render(bjobs, device_out=out_buf)
+            walls.append(time.perf_counter() - t3)
+            rays += st.rays
+            ms += st.render_ms
+            log(f"batch step {k}: {st.rays} rays, {st.render_ms:.1f} ms on the device, {walls[-1] * 1e3:.1f} ms wall")
+        out = {
+            "workload": f"{n_jobs} seeded jobs (config-0/1 types), {n_pix} pixels, {n_paths} paths, 10 bounces",
+            "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1),
+            "ms_per_batch": round(ms / steps, 2),
+            "wall_mrays_per_s": round(rays / sum(walls) / 1e6, 1),
+            "tuned": {"stack_entries": tune.stack_entries, "blocks_per_cu": tune.blocks_per_cu,
+                      "streams": tune.streams, "max_paths": tune.max_paths},
+            "tuner_tried": [{"stack_entries": t[0], "blocks_per_cu": t[1], "streams": t[2], "max_paths": t[3],
+                             "mrays_per_s": round(t[4], 1)} for t in tune.tried],
+        }
+        out_buf.free()
+    # The same jobs one pt_render call at a time, each its own scene, resident
+    # and warmed up; device time summed and wall time of the calls.
+    scenes = [pt.Scene(g) for g in ptgps]
+    images = [pt.DeviceBuffer(12 * j.width * j.height) for j in jobs]
+    for s, j, im in zip(scenes, jobs, images):
+        s.render(j.width, j.height, j.spp, seed=j.seed, device_out=im)
+    rays, ms = 0, 0.0
+    t4 = time.perf_counter()
+    for s, j, im in zip(scenes, jobs, images):
+        _, _, st = s.render(j.width, j.height, j.spp, seed=j.seed, device_out=im)
+        rays += st.rays
+        ms += st.render_ms
+    wall = time.perf_counter() - t4
+    for s, im in zip(scenes, images):
+        im.free()
+        s.close()
+    out["one_call_per_job_mrays_per_s"] = round(rays / (ms * 1e-3) / 1e6, 1)
+    out["one_call_per_job_wall_mrays_per_s"] = round(rays / wall / 1e6, 1)
+    log(f"batch: {out['mrays_per_s']} Mrays/s batched ({out['wall_mrays_per_s']} wall), "
+        f"{out['one_call_per_job_mrays_per_s']} one call per job ({out['one_call_per_job_wall_mrays_per_s']} wall)")
+    return out
+
+
+def guided_leg(pt, ref_spp):
+    """Equal-time relMSE of the guided and plain tracers on the hard-lighting
+    room at 1080p, against a ref_spp plain render."""
+    import numpy as np
+
+    W, H, spp_plain = 1920, 1080, 64
+    t0 = time.perf_counter()
+    ptgp = pt.pack(pt.generate(4, 1, 1.0))
+    log(f"guided: room scene packed in {time.perf_counter() - t0:.1f} s")
+
+    def rel_mse(img, ref):
+        return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+    with pt.Scene(ptgp) as s:
+        ref = np.zeros((H, W, 3), np.float64)
+        chunk = 256
+        for k in range(ref_spp // chunk):
+            img, _, st = s.render(W, H, chunk, seed=90000 + k)
+            ref += img
+            log(f"guided: reference {(k + 1) * chunk}/{ref_spp} spp ({st.render_ms:.0f} ms)")
+        ref = (ref / (ref_spp // chunk)).astype(np.float32)
+        plain, _, sp = s.render(W, H, spp_plain, seed=11)
+        e_plain = rel_mse(plain, ref)
+        budget = sp.render_ms
+        g = pt.Guide(s, seed=2, learning_rate=3e-3, steps_per_batch=16)
+        s.set_guide(g)
+        # Training: 1-spp batches with 16 Adam steps after each, for a quarter of the budget.
+        train_ms, n = 0.0, 0
+        while train_ms < 0.25 * budget:
+            _, _, st = s.render(W, H, 1, seed=5000 + n, train=True, samples_per_batch=1)
+            train_ms += st.render_ms
+            n += 1
+        # The guided render's cost per spp, then the rest of the budget.
+        _, _, st1 = s.render(W, H, 4, seed=7, guide=True)
+        per_spp = st1.render_ms / 4
+        spp_g = max(1, int((budget - train_ms) / per_spp))
+        guided, _, sg = s.render(W, H, spp_g, seed=21, guide=True)
+        e_guided = rel_mse(guided, ref)
+        guided_eq, _, _ = s.render(W, H, spp_plain, seed=41, guide=True)
+        e_guided_eq = rel_mse(guided_eq, ref)
+        g.close()
+    # relMSE falls as 1/spp: the guided time to reach the plain tracer's relMSE.
+    t_match = train_ms + sg.render_ms * e_guided / e_plain
+    out = {
+        "workload": f"room with a 1M-triangle displaced mesh, lamps behind a 0.5 x 0.5 opening, {W}x{H}; "
+                    f"reference {ref_spp} spp plain",
+        "plain": {"spp": spp_plain, "ms": round(budget, 1), "relmse": round(e_plain, 5)},
+        "guided": {"train_ms": round(train_ms, 1), "train_batches": n, "spp": spp_g, "render_ms": round(sg.render_ms, 1),
+                   "relmse": round(e_guided, 5)},
+        "equal_time_relmse_ratio": round(e_guided / e_plain, 3),
+        "equal_spp_relmse_ratio": round(e_guided_eq / e_plain, 3),
+        "guided_over_plain_ms_per_spp": round(per_spp / (sp.render_ms / spp_plain), 3),
+        "time_to_plain_relmse_ratio": round(t_match / budget, 3),
+        "target_time_ratio": 0.5,
+    }
+    log(f"guided: {json.dumps(out)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +248,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the library's choice)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-workload leg")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided equal-time leg")
+    ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also copy MIB MiB device to device (a kernel of known traffic, to scale HBM counters)")
     args = ap.parse_args()
@@ -196,6 +337,8 @@ def main():
         b.free()
     image.free()
     scene.close()
+    batch = None if args.no_batch else batch_leg(pt, args.batch_jobs, max(1, args.steps))
+    guided = None if args.no_guided else guided_leg(pt, args.guided_ref_spp)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -208,6 +351,8 @@ def main():
         "host_output_mrays_per_s": round(hst.rays / host_wall / 1e6, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batch": batch,
+        "guided": guided,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)

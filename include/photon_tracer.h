@@ -263,11 +263,12 @@ typedef struct {
 pt_status pt_render_batch(pt_batch* batch, const pt_batch_job* jobs, int n_jobs, const pt_batch_params* params,
                           float* rgb, pt_render_stats* stats);
 
-/* The autotuner: renders the jobs at a sixteenth of their spp under candidate
- * launch shapes -- LDS stack entries (8, 16, 32) x traversal blocks per CU,
- * then streams (1, 2, 4), then batch sizes (2^22, 2^24 paths) -- and keeps the
- * fastest in the batch for later renders whose params leave them 0. result
- * (optional): the chosen shape and the measured rate of each candidate. */
+/* The autotuner: renders the jobs (as given: pass fewer samples for a quicker
+ * tune) under candidate launch shapes -- LDS stack entries (8, 16, 32) x
+ * traversal blocks per CU, then streams (1, 2, 4), then batch sizes (2^22,
+ * 2^24 paths) -- and keeps the fastest in the batch for later renders whose
+ * params leave them 0. result (optional): the chosen shape and the measured
+ * rate of each candidate. */
 typedef struct {
     int32_t stack_entries, blocks_per_cu, streams, n_tried;
     uint64_t max_paths;

@@ -433,14 +433,13 @@ pt_status pt_batch_autotune(pt_batch* b, const pt_batch_job* jobs, int n_jobs, i
     if (!b || !jobs || n_jobs < 1) return pt::fail(PT_ERR_INVALID, "pt_batch_autotune: null argument");
     PT_GUARD({
         check_jobs(b, jobs, n_jobs);
-        // The jobs at a sixteenth of their spp (at least 1): the same scenes,
-        // views and image sizes, a short render per candidate.
+        // Each candidate renders the jobs as given: small renders are bound
+        // by launch latency and queue tails and would pick a shape that
+        // starves the full render (a caller wanting a quicker tune passes
+        // jobs with fewer samples).
         std::vector<pt_batch_job> short_jobs(jobs, jobs + n_jobs);
         size_t total = 0;
-        for (auto& j : short_jobs) {
-            j.spp = std::max(1, j.spp / 16);
-            total += size_t(j.width) * size_t(j.height) * 3;
-        }
+        for (auto& j : short_jobs) total += size_t(j.width) * size_t(j.height) * 3;
         std::vector<float> img(total);
         struct cand {
             int S, per_cu, n_streams;
