@@ -176,55 +176,66 @@ def batch_leg(pt, n_jobs, steps):
     return out
 
 
-def guided_leg(pt, ref_spp):
+def guided_leg(pt, ref_spp, spp_plain=256):
     """Equal-time relMSE of the guided and plain tracers on the hard-lighting
-    room at 1080p, against a ref_spp plain render."""
+    room at 1080p. The reference is ref_spp plain samples in two independent
+    halves; relMSE subtracts the reference's own variance (estimated per pixel
+    from the halves), so a reference only a few times the test spp does not
+    pull both tracers' errors together. Budget: the plain render at
+    spp_plain; the guided tracer trains for a quarter of it (1-spp batches,
+    64 Adam steps after each) and renders for the rest."""
     import numpy as np
 
-    W, H, spp_plain = 1920, 1080, 64
+    W, H = 1920, 1080
     t0 = time.perf_counter()
     ptgp = pt.pack(pt.generate(4, 1, 1.0))
     log(f"guided: room scene packed in {time.perf_counter() - t0:.1f} s")
-
-    def rel_mse(img, ref):
-        return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
-
+    chunk = 256
     with pt.Scene(ptgp) as s:
-        ref = np.zeros((H, W, 3), np.float64)
-        chunk = 256
-        for k in range(ref_spp // chunk):
-            img, _, st = s.render(W, H, chunk, seed=90000 + k)
-            ref += img
-            log(f"guided: reference {(k + 1) * chunk}/{ref_spp} spp ({st.render_ms:.0f} ms)")
-        ref = (ref / (ref_spp // chunk)).astype(np.float32)
+        halves = []
+        for h in range(2):
+            acc = np.zeros((H, W, 3), np.float64)
+            n = max(1, ref_spp // (2 * chunk))
+            for k in range(n):
+                img, _, st = s.render(W, H, chunk, seed=90000 + 1000 * h + k)
+                acc += img
+            halves.append(acc / n)
+            log(f"guided: reference half {h}: {n * chunk} spp ({st.render_ms:.0f} ms a chunk)")
+        ref = 0.5 * (halves[0] + halves[1])
+        ref_var = 0.25 * (halves[0] - halves[1]) ** 2   # per pixel: Var of the mean of the two halves
+        den = ref ** 2 + 0.01
+
+        def rel_mse(img):
+            return float(np.mean(((img - ref) ** 2 - ref_var) / den))
+
         plain, _, sp = s.render(W, H, spp_plain, seed=11)
-        e_plain = rel_mse(plain, ref)
+        e_plain = rel_mse(plain)
         budget = sp.render_ms
-        g = pt.Guide(s, seed=2, learning_rate=3e-3, steps_per_batch=16)
+        g = pt.Guide(s, seed=2, learning_rate=3e-3, steps_per_batch=64)
         s.set_guide(g)
-        # Training: 1-spp batches with 16 Adam steps after each, for a quarter of the budget.
         train_ms, n = 0.0, 0
         while train_ms < 0.25 * budget:
             _, _, st = s.render(W, H, 1, seed=5000 + n, train=True, samples_per_batch=1)
             train_ms += st.render_ms
             n += 1
-        # The guided render's cost per spp, then the rest of the budget.
-        _, _, st1 = s.render(W, H, 4, seed=7, guide=True)
-        per_spp = st1.render_ms / 4
+        steps, used, loss = g.stats()
+        # The guided render's cost per spp (a 16-spp render), then the rest of the budget.
+        _, _, st1 = s.render(W, H, 16, seed=7, guide=True)
+        per_spp = st1.render_ms / 16
         spp_g = max(1, int((budget - train_ms) / per_spp))
         guided, _, sg = s.render(W, H, spp_g, seed=21, guide=True)
-        e_guided = rel_mse(guided, ref)
+        e_guided = rel_mse(guided)
         guided_eq, _, _ = s.render(W, H, spp_plain, seed=41, guide=True)
-        e_guided_eq = rel_mse(guided_eq, ref)
+        e_guided_eq = rel_mse(guided_eq)
         g.close()
     # relMSE falls as 1/spp: the guided time to reach the plain tracer's relMSE.
     t_match = train_ms + sg.render_ms * e_guided / e_plain
     out = {
         "workload": f"room with a 1M-triangle displaced mesh, lamps behind a 0.5 x 0.5 opening, {W}x{H}; "
-                    f"reference {ref_spp} spp plain",
-        "plain": {"spp": spp_plain, "ms": round(budget, 1), "relmse": round(e_plain, 5)},
-        "guided": {"train_ms": round(train_ms, 1), "train_batches": n, "spp": spp_g, "render_ms": round(sg.render_ms, 1),
-                   "relmse": round(e_guided, 5)},
+                    f"reference {ref_spp} spp plain in two halves (relMSE less the reference's variance)",
+        "plain": {"spp": spp_plain, "ms": round(budget, 1), "relmse": round(e_plain, 6)},
+        "guided": {"train_ms": round(train_ms, 1), "train_batches": n, "adam_steps": steps, "spp": spp_g,
+                   "render_ms": round(sg.render_ms, 1), "relmse": round(e_guided, 6)},
         "equal_time_relmse_ratio": round(e_guided / e_plain, 3),
         "equal_spp_relmse_ratio": round(e_guided_eq / e_plain, 3),
         "guided_over_plain_ms_per_spp": round(per_spp / (sp.render_ms / spp_plain), 3),
@@ -232,6 +243,26 @@ def guided_leg(pt, ref_spp):
         "target_time_ratio": 0.5,
     }
     log(f"guided: {json.dumps(out)}")
+    return out
+
+
+def guided_kernels(pt):
+    """Plain and guided 16-spp 1080p renders of the room, twice each, after
+    four training batches: the kernels a profiler should see per render."""
+    ptgp = pt.pack(pt.generate(4, 1, 1.0))
+    W, H, out = 1920, 1080, {}
+    with pt.Scene(ptgp) as s:
+        g = pt.Guide(s, seed=2, learning_rate=3e-3, steps_per_batch=16)
+        s.set_guide(g)
+        for k in range(4):
+            _, _, st = s.render(W, H, 1, seed=100 + k, train=True, samples_per_batch=1)
+        for k in range(2):
+            _, _, sp = s.render(W, H, 16, seed=1)
+            _, _, sg = s.render(W, H, 16, seed=2, guide=True)
+            log(f"room 1080p 16 spp: plain {sp.render_ms:.1f} ms, guided {sg.render_ms:.1f} ms")
+        out = {"plain_ms": round(sp.render_ms, 2), "guided_ms": round(sg.render_ms, 2),
+               "plain_rays": sp.rays, "guided_rays": sg.rays}
+        g.close()
     return out
 
 
@@ -252,6 +283,8 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--no-guided", action="store_true", help="skip the guided equal-time leg")
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
+    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels"], default=None,
+                    help="run one leg alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also copy MIB MiB device to device (a kernel of known traffic, to scale HBM counters)")
     args = ap.parse_args()
@@ -261,6 +294,13 @@ def main():
         sys.exit(2)
 
     import photon_tracer_amd as pt
+
+    if args.only:
+        leg = {"batch": lambda: batch_leg(pt, args.batch_jobs, max(1, args.steps)),
+               "guided": lambda: guided_leg(pt, args.guided_ref_spp),
+               "guided-kernels": lambda: guided_kernels(pt)}[args.only]
+        print(json.dumps({args.only: leg()}), flush=True)
+        return
 
     t0 = time.perf_counter()
     ptsd = pt.generate(1, 1, args.scale)
