@@ -109,7 +109,7 @@ def cpu_baseline(ptsd, width, height, spp_full, seconds):
     }
 
 
-def batch_leg(pt, n_jobs, steps):
+def batch_leg(pt, n_jobs, steps, tune=True, graph=True):
     """The batched workload: n_jobs seeded jobs, generated and packed on host
     threads, merged into one device scene, tuned, then rendered in one call
     per step; the same jobs one pt_render call at a time for comparison."""
@@ -129,15 +129,16 @@ def batch_leg(pt, n_jobs, steps):
     out = {}
     with pt.Batch(ptgps) as b:
         t2 = time.perf_counter()
-        tune = b.autotune(bjobs)
-        log(f"batch: autotuned in {time.perf_counter() - t2:.1f} s: stack {tune.stack_entries}, "
-            f"blocks/CU {tune.blocks_per_cu}, streams {tune.streams}, paths {tune.max_paths}")
+        tuned = b.autotune(bjobs) if tune else None
+        if tuned:
+            log(f"batch: autotuned in {time.perf_counter() - t2:.1f} s: stack {tuned.stack_entries}, "
+                f"blocks/CU {tuned.blocks_per_cu}, streams {tuned.streams}, paths {tuned.max_paths}")
         out_buf = pt.DeviceBuffer(12 * n_pix)   # the images stay in HBM, as in the timed frames above
-        _, st = b.render(bjobs, device_out=out_buf)   # warmup (captures the graphs)
+        _, st = b.render(bjobs, device_out=out_buf, graph=graph)   # warmup (captures the graphs)
         rays, ms, walls = 0, 0.0, []
         for k in range(steps):
             t3 = time.perf_counter()
-            _, st = b.render(bjobs, device_out=out_buf)
+            _, st = b.render(bjobs, device_out=out_buf, graph=graph)
             walls.append(time.perf_counter() - t3)
             rays += st.rays
             ms += st.render_ms
@@ -147,11 +148,12 @@ def batch_leg(pt, n_jobs, steps):
             "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1),
             "ms_per_batch": round(ms / steps, 2),
             "wall_mrays_per_s": round(rays / sum(walls) / 1e6, 1),
-            "tuned": {"stack_entries": tune.stack_entries, "blocks_per_cu": tune.blocks_per_cu,
-                      "streams": tune.streams, "max_paths": tune.max_paths},
-            "tuner_tried": [{"stack_entries": t[0], "blocks_per_cu": t[1], "streams": t[2], "max_paths": t[3],
-                             "mrays_per_s": round(t[4], 1)} for t in tune.tried],
         }
+        if tuned:
+            out["tuned"] = {"stack_entries": tuned.stack_entries, "blocks_per_cu": tuned.blocks_per_cu,
+                            "streams": tuned.streams, "max_paths": tuned.max_paths}
+            out["tuner_tried"] = [{"stack_entries": t[0], "blocks_per_cu": t[1], "streams": t[2], "max_paths": t[3],
+                                   "mrays_per_s": round(t[4], 1)} for t in tuned.tried]
         out_buf.free()
     # The same jobs one pt_render call at a time, each its own scene, resident
     # and warmed up; device time summed and wall time of the calls.
@@ -281,6 +283,8 @@ def main():
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the library's choice)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-workload leg")
     ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batch-no-tune", action="store_true", help="batch leg with the default launch shape")
+    ap.add_argument("--batch-eager", action="store_true", help="batch leg without captured graphs")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided equal-time leg")
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--only", choices=["batch", "guided", "guided-kernels"], default=None,
@@ -296,7 +300,8 @@ def main():
     import photon_tracer_amd as pt
 
     if args.only:
-        leg = {"batch": lambda: batch_leg(pt, args.batch_jobs, max(1, args.steps)),
+        leg = {"batch": lambda: batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
+                                          not args.batch_eager),
                "guided": lambda: guided_leg(pt, args.guided_ref_spp),
                "guided-kernels": lambda: guided_kernels(pt)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
@@ -377,7 +382,8 @@ def main():
         b.free()
     image.free()
     scene.close()
-    batch = None if args.no_batch else batch_leg(pt, args.batch_jobs, max(1, args.steps))
+    batch = None if args.no_batch else batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
+                                                  not args.batch_eager)
     guided = None if args.no_guided else guided_leg(pt, args.guided_ref_spp)
 
     line = {

@@ -156,7 +156,9 @@ render_shape shape_for(pt_batch* b, const pt_batch_params& P, const pt_batch_job
     render_shape r{};
     pt_scene* s = b->scene;
     r.S = ptapi::stack_entries_for(s, P.stack_entries > 0 ? P.stack_entries : b->tuned_stack);
-    r.per_cu = P.blocks_per_cu > 0 ? P.blocks_per_cu : b->tuned_blocks > 0 ? b->tuned_blocks : ptd::trace_blocks_per_cu(r.S);
+    r.per_cu = P.blocks_per_cu > 0 ? P.blocks_per_cu
+               : b->tuned_blocks > 0 ? b->tuned_blocks
+                                      : ptd::path_blocks_per_cu(r.S, false);
     if (r.per_cu > 64) throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: blocks_per_cu above 64");
     r.n_streams = P.streams > 0 ? P.streams : b->tuned_streams > 0 ? b->tuned_streams : std::min(2, n_jobs);
     r.n_streams = std::max(1, std::min({r.n_streams, n_jobs, kMaxStreams}));
@@ -468,7 +470,7 @@ pt_status pt_batch_autotune(pt_batch* b, const pt_batch_job* jobs, int n_jobs, i
         };
         // Stage 1: LDS stack entries x traversal blocks per CU, one stream.
         for (int S : {8, 16, 32}) {
-            const int occ = ptd::trace_blocks_per_cu(S);
+            const int occ = ptd::path_blocks_per_cu(S, false);
             std::vector<int> blocks{occ};
             if (occ > 2) blocks.push_back(occ - 1);
             if (occ > 4) blocks.push_back(occ / 2);

@@ -84,8 +84,10 @@ constexpr uint32_t GUIDE_TARGET_BLOCK = 256;
 // counts: null, or six 64-bit visit totals (Visits) to add the batch's into.
 hipError_t launch_trace(const SceneView& sc, const float4* rays, uint32_t n, Hit* hits, bool any_hit, uint2* spill,
                         uint32_t* work, unsigned long long* counts, int stack_entries, int grid, hipStream_t stream);
-// Occupancy of the traversal kernels: blocks of 256 per CU.
+// Occupancy of the traversal kernels: blocks of 256 per CU (pt_trace's
+// kernel, and the path loop's traversal stage).
 int trace_blocks_per_cu(int stack_entries);
+int path_blocks_per_cu(int stack_entries, bool inst);
 constexpr int TRACE_BLOCK = 256;
 
 // The bounce loop of one batch of paths. ev: null, or 2 * depth events

@@ -1113,6 +1113,23 @@ hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStre
     return hipGetLastError();
 }
 
+template <int S, bool INST>
+static int extend_occupancy() {
+    int blocks = 0;
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_extend<S, false, INST, false>, TRACE_BLOCK, 0);
+    return e == hipSuccess && blocks > 0 ? blocks : 1;
+}
+
+int path_blocks_per_cu(int stack_entries, bool inst) {
+    switch (stack_entries) {
+        case 8: return inst ? extend_occupancy<8, true>() : extend_occupancy<8, false>();
+        case 16: return inst ? extend_occupancy<16, true>() : extend_occupancy<16, false>();
+        case 32: return inst ? extend_occupancy<32, true>() : extend_occupancy<32, false>();
+        default: return 1;
+    }
+}
+
 hipError_t launch_clear_frame(const FrameParams& fp, hipStream_t stream) {
     k_clear_frame<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp);
     return hipGetLastError();
