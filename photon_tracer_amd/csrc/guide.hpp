@@ -128,14 +128,30 @@ __device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
 }
 __device__ __forceinline__ float vmf_pdf(const Lobe& l, f3 w) { return vmf_pdf(l.mu, l.kappa, w); }
 
+// The mixture as the path loop samples and evaluates it (and pt_guide_pdf /
+// pt_guide_sample): the hardware's fast exponential, logarithm, sine and
+// cosine (arguments here are bounded: angles in (0, 2 pi), exponents at most
+// 2 e^10), where the trainer's gradient keeps the correctly rounded library
+// functions. The sampler and the pdf share these functions and the lobe axis
+// is renormalised, so a guided sample is weighted by the density it was
+// drawn from, and the estimator stays unbiased.
+__device__ __forceinline__ float fast_sigmoid(float v) { return __fdividef(1.0f, 1.0f + __expf(-v)); }
+
+__device__ __forceinline__ float fast_vmf_pdf(f3 mu, float kappa, f3 w) {
+    return __fdividef(kappa, 2 * PI_F * (1.0f - __expf(-2 * kappa))) * __expf(kappa * (dot(mu, w) - 1));
+}
+
 // Lobe k's axis and concentration from the outputs o (FP32 or FP16).
 template <class T>
 __device__ __forceinline__ void lobe_at(const T* o, int k, f3& mu, float& kappa) {
-    const float th = PI_F * sigmoid(static_cast<float>(o[4 * k]));
-    const float ph = 2 * PI_F * sigmoid(static_cast<float>(o[4 * k + 1]));
-    const float st = sinf(th);
-    mu = mk(st * cosf(ph), st * sinf(ph), cosf(th));
-    kappa = expf(fminf(fmaxf(static_cast<float>(o[4 * k + 2]), LOG_KAPPA_MIN), LOG_KAPPA_MAX));
+    const float th = PI_F * fast_sigmoid(static_cast<float>(o[4 * k]));
+    const float ph = 2 * PI_F * fast_sigmoid(static_cast<float>(o[4 * k + 1]));
+    float st, ct, sp, cp;
+    __sincosf(th, &st, &ct);
+    __sincosf(ph, &sp, &cp);
+    const f3 m = mk(st * cp, st * sp, ct);
+    mu = m * rsqrtf(dot(m, m));
+    kappa = __expf(fminf(fmaxf(static_cast<float>(o[4 * k + 2]), LOG_KAPPA_MIN), LOG_KAPPA_MAX));
 }
 
 // The softmax of the weight logits as (max, sum of exp(logit - max)).
@@ -146,7 +162,7 @@ __device__ __forceinline__ void weight_norm(const T* o, float& mx, float& sum) {
     for (int k = 1; k < LOBES; ++k) mx = fmaxf(mx, static_cast<float>(o[4 * k + 3]));
     sum = 0;
 #pragma unroll 1
-    for (int k = 0; k < LOBES; ++k) sum += expf(static_cast<float>(o[4 * k + 3]) - mx);
+    for (int k = 0; k < LOBES; ++k) sum += __expf(static_cast<float>(o[4 * k + 3]) - mx);
 }
 
 // Mixture pdf at w, one lobe at a time (the loops stay rolled, so few
@@ -161,9 +177,9 @@ __device__ __forceinline__ float mixture_pdf(const T* o, f3 w) {
         f3 mu;
         float kappa;
         lobe_at(o, k, mu, kappa);
-        p += expf(static_cast<float>(o[4 * k + 3]) - mx) * vmf_pdf(mu, kappa, w);
+        p += __expf(static_cast<float>(o[4 * k + 3]) - mx) * fast_vmf_pdf(mu, kappa, w);
     }
-    return p / sum;
+    return __fdividef(p, sum);
 }
 
 // A direction from the mixture: u0 picks the lobe by weight (the first whose
@@ -180,7 +196,7 @@ __device__ __forceinline__ f3 mixture_sample(const T* o, float u0, float u1, flo
     bool chosen = false;
 #pragma unroll 1
     for (int k = 0; k < LOBES; ++k) {
-        acc += expf(static_cast<float>(o[4 * k + 3]) - mx);
+        acc += __expf(static_cast<float>(o[4 * k + 3]) - mx);
         if (!chosen && target < acc) {
             sel = k;
             chosen = true;
@@ -189,14 +205,16 @@ __device__ __forceinline__ f3 mixture_sample(const T* o, float u0, float u1, flo
     f3 nn;
     float kappa;
     lobe_at(o, sel, nn, kappa);
-    const float cw = fminf(1.0f, fmaxf(-1.0f, 1 + logf(u1 + (1 - u1) * expf(-2 * kappa)) / kappa));
+    const float cw = fminf(1.0f, fmaxf(-1.0f, 1 + __fdividef(__logf(u1 + (1 - u1) * __expf(-2 * kappa)), kappa)));
     const float s = sqrtf(fmaxf(0.0f, 1 - cw * cw));
     const float phi = 2 * PI_F * u2;
+    float sphi, cphi;
+    __sincosf(phi, &sphi, &cphi);
     const float sign = copysignf(1.0f, nn.z);
     const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
     const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
     const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
-    return t1 * (s * cosf(phi)) + t2 * (s * sinf(phi)) + nn * cw;
+    return t1 * (s * cphi) + t2 * (s * sphi) + nn * cw;
 }
 
 // Gradient of the sample's loss ws * -log p(w) with respect to the 32

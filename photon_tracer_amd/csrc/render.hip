@@ -637,6 +637,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1, nxt = cur ^ 1;
     __shared__ BlockAppend<3> sh;
+    // GUIDED: each thread's 32 network outputs, copied from HBM with four
+    // 16-byte loads into an LDS row of 17 words (an odd stride: the lobe loops'
+    // reads, one output at a time, fall in distinct banks), instead of
+    // gathering them two bytes at a time from HBM inside the lobe loops.
+    constexpr int GROW = guide::OUT / 2 + 1;
+    __shared__ uint32_t gtile[GUIDED ? SHADE_BLOCK * GROW : 1];
     uint32_t* const counters[3] = {&fp.ctrl[b + 1].n_rays, &c->n_shadow, &c->n_backface};
     const uint32_t stride = gridDim.x * SHADE_BLOCK;
     // The block steps through the queue together (block_append synchronises).
@@ -646,6 +652,20 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
         r.cont = r.shadow = r.backface = false;
         uint32_t p = 0;
         if (q < n) {
+            const _Float16* go = nullptr;
+            if constexpr (GUIDED) {
+                const uint4* src = reinterpret_cast<const uint4*>(fp.gout + static_cast<size_t>(q) * guide::OUT);
+                uint32_t* row = gtile + threadIdx.x * GROW;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint4 v = src[i];
+                    row[4 * i] = v.x;
+                    row[4 * i + 1] = v.y;
+                    row[4 * i + 2] = v.z;
+                    row[4 * i + 3] = v.w;
+                }
+                go = reinterpret_cast<const _Float16*>(row);
+            }
             const Hit h = fp.hits[q];
             f3 o, d;
             PathState ps;
@@ -675,7 +695,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 o = xyz(o4);
                 d = xyz(d4);
             }
-            r = shade_hit<GUIDED>(fp, b, p, h, o, d, ps, GUIDED ? fp.gout + static_cast<size_t>(q) * guide::OUT : nullptr);
+            r = shade_hit<GUIDED>(fp, b, p, h, o, d, ps, go);
             // The radiance always (accumulate reads it); the rest only for a
             // path that goes on.
             fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, path_flags(ps));
