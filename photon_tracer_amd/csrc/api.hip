@@ -223,13 +223,6 @@ int trace_grid_for(pt_scene* s, int S) {
     return std::max(1, s->cus * per_cu);
 }
 
-int path_grid_for(pt_scene* s, int S) {
-    // The traversal stage's own occupancy: a grid of more blocks than can be
-    // resident only adds late starters to the persistent kernel's tail.
-    const int per_cu = s->tuned_blocks > 0 ? s->tuned_blocks : ptd::path_blocks_per_cu(S, s->view.inst_root != NONE);
-    return std::max(1, s->cus * per_cu);
-}
-
 void ensure_spill(pt_scene* s, int grid) {
     const size_t lanes = size_t(grid) * ptd::TRACE_BLOCK;
     s->spill.ensure(lanes * size_t(s->max_stack) * sizeof(uint2));
@@ -421,7 +414,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         const size_t paths = npix * size_t(spb);
         if (paths >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render: batch too large");
         const int S = stack_entries_for(s, P.stack_entries);
-        const int trace_grid = ptapi::path_grid_for(s, S);
+        const int trace_grid = trace_grid_for(s, S);
         const int shade_grid = std::max(1, s->cus * 2);  // blocks of 1024
         ensure_spill(s, trace_grid);
         const bool want_aux = aux != nullptr;

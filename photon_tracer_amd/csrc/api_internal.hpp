@@ -146,10 +146,8 @@ void check_ptgp(const uint32_t* w, size_t n);
 // LDS stack entries for a render (8, 16 or 32): requested, else tuned, else
 // the default for the scene.
 int stack_entries_for(pt_scene* s, int requested);
-// Traversal grid: blocks per CU (tuned, else the occupancy limit) x CUs, for
-// pt_trace's kernel and for the path loop's traversal stage.
+// Traversal grid: blocks per CU (tuned, else the occupancy limit) x CUs.
 int trace_grid_for(pt_scene* s, int S);
-int path_grid_for(pt_scene* s, int S);
 // The traversal stacks' spill area for a grid of that many blocks.
 void ensure_spill(pt_scene* s, int grid);
 }  // namespace ptapi
