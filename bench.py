@@ -178,14 +178,14 @@ def batch_leg(pt, n_jobs, steps, tune=True, graph=True):
     return out
 
 
-def guided_leg(pt, ref_spp, spp_plain=256):
+def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128):
     """Equal-time relMSE of the guided and plain tracers on the hard-lighting
     room at 1080p. The reference is ref_spp plain samples in two independent
     halves; relMSE subtracts the reference's own variance (estimated per pixel
     from the halves), so a reference only a few times the test spp does not
     pull both tracers' errors together. Budget: the plain render at
     spp_plain; the guided tracer trains for a quarter of it (1-spp batches,
-    64 Adam steps after each) and renders for the rest."""
+    steps_per_batch Adam steps after each) and renders for the rest."""
     import numpy as np
 
     W, H = 1920, 1080
@@ -213,7 +213,7 @@ def guided_leg(pt, ref_spp, spp_plain=256):
         plain, _, sp = s.render(W, H, spp_plain, seed=11)
         e_plain = rel_mse(plain)
         budget = sp.render_ms
-        g = pt.Guide(s, seed=2, learning_rate=3e-3, steps_per_batch=64)
+        g = pt.Guide(s, seed=2, learning_rate=3e-3, steps_per_batch=steps_per_batch)
         s.set_guide(g)
         train_ms, n = 0.0, 0
         while train_ms < 0.25 * budget:
@@ -221,11 +221,13 @@ def guided_leg(pt, ref_spp, spp_plain=256):
             train_ms += st.render_ms
             n += 1
         steps, used, loss = g.stats()
-        # The guided render's cost per spp (a 16-spp render), then the rest of the budget.
+        # The guided render's cost per spp (a 16-spp render), then the rest of
+        # the budget in whole 16-spp batches (a sample count with few divisors
+        # would split into many small batches).
         _, _, st1 = s.render(W, H, 16, seed=7, guide=True)
         per_spp = st1.render_ms / 16
-        spp_g = max(1, int((budget - train_ms) / per_spp))
-        guided, _, sg = s.render(W, H, spp_g, seed=21, guide=True)
+        spp_g = max(16, int((budget - train_ms) / per_spp) // 16 * 16)
+        guided, _, sg = s.render(W, H, spp_g, seed=21, guide=True, samples_per_batch=16)
         e_guided = rel_mse(guided)
         guided_eq, _, _ = s.render(W, H, spp_plain, seed=41, guide=True)
         e_guided_eq = rel_mse(guided_eq)
@@ -287,6 +289,8 @@ def main():
     ap.add_argument("--batch-eager", action="store_true", help="batch leg without captured graphs")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided equal-time leg")
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
+    ap.add_argument("--guided-plain-spp", type=int, default=1024,
+                    help="the equal-time budget: the plain render at this many spp")
     ap.add_argument("--only", choices=["batch", "guided", "guided-kernels"], default=None,
                     help="run one leg alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -302,7 +306,7 @@ def main():
     if args.only:
         leg = {"batch": lambda: batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
                                           not args.batch_eager),
-               "guided": lambda: guided_leg(pt, args.guided_ref_spp),
+               "guided": lambda: guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp),
                "guided-kernels": lambda: guided_kernels(pt)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
@@ -384,7 +388,7 @@ def main():
     scene.close()
     batch = None if args.no_batch else batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
                                                   not args.batch_eager)
-    guided = None if args.no_guided else guided_leg(pt, args.guided_ref_spp)
+    guided = None if args.no_guided else guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,

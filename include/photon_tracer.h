@@ -304,7 +304,7 @@ typedef struct {
     int32_t minibatch;          /* samples per Adam step; 0 means 16384 */
     float weight_clamp;         /* training weights are clamped at this many times their batch mean; 0 means 16 */
     int32_t bounces;            /* diffuse vertices of bounces [0, bounces) sample the guide (later ones the
-                                   cosine lobe alone, and the tail launch may take them); 0 means 2 */
+                                   cosine lobe alone, and the tail launch may take them); 0 means every bounce */
 } pt_guide_params;
 
 pt_status pt_guide_create(pt_scene* scene, const pt_guide_params* params, pt_guide** guide);

@@ -409,7 +409,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         if (guided && !G) throw pt::status_error(PT_ERR_INVALID, "pt_render: guided render without a guide (pt_scene_set_guide)");
         // Per path: the guide's outputs; while training, its records, their
         // weights and the compacted samples.
-        const size_t guide_bytes = (guided ? 64 : 0) + (training ? ptd::GUIDE_RECORDS * (64 + 4 + 48) : 0);
+        const size_t guide_bytes = (guided ? 64 + 4 : 0) + (training ? ptd::GUIDE_RECORDS * (64 + 4 + 48) : 0);
         const int spb = choose_spb(P.spp, npix, P.samples_per_batch, free_bytes, guide_bytes);
         const size_t paths = npix * size_t(spb);
         if (paths >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render: batch too large");
@@ -489,6 +489,8 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             fp.guide_bounces = G->bounces;
             B.gout.ensure(paths * ptd::guide::OUT * 2);
             fp.gout = B.gout.as<_Float16>();
+            B.gmis.ensure(paths * 4);
+            fp.gmis = B.gmis.as<float>();
             if (training) {
                 const size_t nrec = paths * ptd::GUIDE_RECORDS;
                 const size_t nblk = (nrec + ptd::GUIDE_TARGET_BLOCK - 1) / ptd::GUIDE_TARGET_BLOCK;

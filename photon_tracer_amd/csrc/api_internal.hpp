@@ -51,10 +51,10 @@ inline constexpr size_t kPathBytes = 2 * 36 + 24 + 48 + 48;
 struct render_bufs {
     dbuf ro[2], rd[2], rpath[2], hits, so, sd, sl, pa, pb, pkey, pq, paux;
     dbuf accum, aux_accum, ctrl, frame, totals, out_rgb, out_aux;
-    dbuf gout, grec, gwt, gblk, gblk_sum, gtot, gsamples;   // guided and training renders
+    dbuf gout, gmis, grec, gwt, gblk, gblk_sum, gtot, gsamples;   // guided and training renders
     std::vector<dbuf*> all() {
         return {&ro[0], &ro[1], &rd[0], &rd[1], &rpath[0], &rpath[1], &hits, &so, &sd, &sl, &pa, &pb, &pkey, &pq,
-                &paux, &accum, &aux_accum, &ctrl, &frame, &totals, &out_rgb, &out_aux, &gout, &grec, &gwt, &gblk,
+                &paux, &accum, &aux_accum, &ctrl, &frame, &totals, &out_rgb, &out_aux, &gout, &gmis, &grec, &gwt, &gblk,
                 &gblk_sum, &gtot, &gsamples};
     }
     void release() {
@@ -116,7 +116,7 @@ struct pt_guide {
     float lr = 1e-3f;
     int steps_per_batch = 0, minibatch = 16384;
     float weight_clamp = 16.0f;  // training weights are clamped at this many times their batch mean
-    int bounces = 2;             // diffuse vertices of the first `bounces` bounces sample the guide
+    int bounces = ptd::MAX_BOUNCES;   // diffuse vertices of the first `bounces` bounces sample the guide
     uint32_t step = 0;           // Adam's step count
     uint64_t stat_steps = 0, stat_samples = 0;
     float stat_loss = 0;

@@ -168,7 +168,7 @@ pt_status pt_guide_create(pt_scene* scene, const pt_guide_params* params, pt_gui
         g->minibatch = p.minibatch > 0 ? p.minibatch : 16384;
         g->weight_clamp = p.weight_clamp > 0 ? p.weight_clamp : 16.0f;
         if (p.bounces < 0 || p.bounces > 63) throw pt::status_error(PT_ERR_INVALID, "guide bounces must be in [0, 63]");
-        g->bounces = p.bounces > 0 ? p.bounces : 2;
+        g->bounces = p.bounces > 0 ? p.bounces : ptd::MAX_BOUNCES;
         const bool box_given = p.box_lo[0] != 0 || p.box_lo[1] != 0 || p.box_lo[2] != 0 || p.box_hi[0] != 0 ||
                                p.box_hi[1] != 0 || p.box_hi[2] != 0;
         if (box_given) {

@@ -75,6 +75,7 @@ struct FrameParams {
     float4* gsamples;         // compacted training samples, 3 float4 each
     float gclamp;             // training weights are clamped at gclamp x their batch mean
     int guide_bounces;        // diffuse vertices of bounces [0, guide_bounces) sample the guide
+    float* gmis;              // guided renders: per path, the product of its guided MIS factors
 };
 constexpr uint32_t GUIDE_RECORDS = 2;   // diffuse vertices per path recorded for training
 constexpr uint32_t GUIDE_TARGET_BLOCK = 256;

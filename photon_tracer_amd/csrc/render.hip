@@ -314,6 +314,7 @@ struct PathState {
     uint64_t key;       // the path's random stream
     uint2 pq;           // QMC pixel seed and sample index
     uint32_t nrec;      // training records written (guide training)
+    float mis;          // guided renders: the product of the guided MIS factors in thr
 };
 
 // The path's flag word (pb[p].w): bit 0 `sampled`, bits 1-2 the records written.
@@ -472,7 +473,9 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
             const float cosv = dot(dir, nrm);
             if (cosv > 0) {
                 const float pc = cosv * (1 / PI_F);
-                wgt = att * (pc / (0.5f * (pc + guide::mixture_pdf(go, dir))));
+                const float f = pc / (0.5f * (pc + guide::mixture_pdf(go, dir)));
+                wgt = att * f;
+                ps.mis *= f;
                 if (fp.guide_mode == 2 && ps.nrec < GUIDE_RECORDS) {
                     record_vertex(fp, p, ps.nrec, b, pnt, nrm, dir, ps.thr, ps.rad);
                     ++ps.nrec;
@@ -559,7 +562,16 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
     if (type != 0) wgt = att;
     ps.thr = mul(ps.thr, wgt);
     if (b + 1 >= 3) {  // roulette_after (renderer.hpp:22,140-146)
-        const float pm = fmaxf(ps.thr.x, fmaxf(ps.thr.y, ps.thr.z));
+        float pm = fmaxf(ps.thr.x, fmaxf(ps.thr.y, ps.thr.z));
+        // A guided render's paths survive at least as often as the cosine
+        // lobe's would: the guided MIS factors that lowered the throughput are
+        // divided back out of the survival probability (factors above 1 are
+        // not, so no path survives less often than without the guide). A path
+        // the guide sent toward bright light has a small weight, and a roulette
+        // on that weight would kill most such paths and scale the few
+        // survivors up, putting back the variance guiding removed. Any
+        // survival probability divided out keeps the estimator unbiased.
+        if constexpr (GUIDED) pm = pm / fminf(ps.mis, 1.0f);
         if (pm < 1) {
             if (rand01(ps.key, ps.cnt) >= pm) return r;
             ps.thr = ps.thr * (1 / pm);
@@ -653,7 +665,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
         uint32_t p = 0;
         if (q < n) {
             const _Float16* go = nullptr;
-            if constexpr (GUIDED) {
+            if (GUIDED && b < fp.guide_bounces) {
                 const uint4* src = reinterpret_cast<const uint4*>(fp.gout + static_cast<size_t>(q) * guide::OUT);
                 uint32_t* row = gtile + threadIdx.x * GROW;
 #pragma unroll
@@ -679,6 +691,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 ps.cnt = 0;
                 ps.sampled = false;
                 ps.nrec = 0;
+                ps.mis = 1;
                 ps.key = path_key(fc.seed, cs.i, cs.j, cs.index);
                 ps.pq = make_uint2(cs.pseed, cs.index);
             } else {
@@ -692,6 +705,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 ps.rad = xyz(B);
                 ps.cnt = __float_as_uint(A.w);
                 read_flags(B.w, ps);
+                ps.mis = GUIDED ? fp.gmis[p] : 1.0f;
                 o = xyz(o4);
                 d = xyz(d4);
             }
@@ -705,6 +719,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                     fp.pkey[p] = make_uint2(static_cast<uint32_t>(ps.key), static_cast<uint32_t>(ps.key >> 32));
                     fp.pq[p] = ps.pq;
                 }
+                if (GUIDED) fp.gmis[p] = ps.mis;
             }
         }
         // Every thread of the block reaches the appends.
@@ -762,7 +777,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail_shadow(FrameParams fp, int
 // lane: closest hit, shade, light sample, next bounce. The later bounces'
 // stage launches then return at once. Rays are counted as the stages count
 // them, into ctrl[MAX_BOUNCES].
-template <int S, bool INST>
+template <int S, bool INST, bool GUIDED = false>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
     if (!tail_starts(fp, b)) return;
     Ctrl* c = fp.ctrl + b;
@@ -790,13 +805,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
             ps.rad = xyz(B);
             ps.cnt = __float_as_uint(A.w);
             read_flags(B.w, ps);
+            ps.mis = GUIDED ? fp.gmis[p] : 1.0f;
             f3 o = xyz(o4), d = xyz(d4);
             float tmin = o4.w, tmax = d4.w;
             for (int bb = b; bb < fp.depth; ++bb) {
                 Hit h;
                 trace_ray<false, false, INST>(fp.sc, o, d, tmin, tmax, h, st);
                 if (bb > b) ++rays;  // bounce b's rays were counted when they were queued
-                const ShadeResult r = shade_hit(fp, bb, p, h, o, d, ps, nullptr);   // never with the guide
+                const ShadeResult r = shade_hit<GUIDED>(fp, bb, p, h, o, d, ps, nullptr);   // never samples the guide
                 if (r.backface) ++backfaces;
                 if (r.shadow) {
                     ++shadows;
@@ -958,10 +974,15 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams fp, float* rgb, flo
 template <int S, bool INST>
 static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream, hipEvent_t* ev) {
     const int acc_grid = shade_grid * 8;
+    // Every bounce of a guided render shades with the guided kernels (which
+    // sample the guide only below guide_bounces), since its roulette divides
+    // the guided MIS factors out of the throughput.
+    const bool guided = fp.guide_mode != 0;
     for (int b = 0; b < fp.depth; ++b) {
         if (b > 0) {
             k_tail_shadow<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
-            k_tail<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            if (guided) k_tail<S, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            else k_tail<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
         }
         if (ev) (void)hipEventRecord(ev[2 * b], stream);
         if (b == 0) {
@@ -977,7 +998,6 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
             if (b == 0) k_guide_infer<true><<<infer_grid, 64, 0, stream>>>(fp, b);
             else k_guide_infer<false><<<infer_grid, 64, 0, stream>>>(fp, b);
         }
-        const bool guided = fp.guide_mode && b < fp.guide_bounces;
         if (b == 0) {
             if (guided) k_shade<true, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
             else k_shade<true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
