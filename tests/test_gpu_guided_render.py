@@ -84,7 +84,7 @@ def test_trained_guide_beats_plain_on_the_hard_room():
     """Config 4 (the lamps hide behind a partition; light reaches the room through a 0.5 x 0.5 opening):
     after online training (one call: 1-spp batches, 16 Adam steps after each), guided renders have
     clearly lower relMSE than plain ones at equal spp, against a 4096-spp plain reference (measured
-    about 0.7x; asserted below 0.85x over four seeds)."""
+    about 0.3x; asserted below 0.5x over four seeds)."""
     ptgp = pt.pack(pt.generate(4, 1, 0.02))
     W, H, spp = 192, 144, 64
     with pt.Scene(ptgp) as s:
@@ -98,4 +98,4 @@ def test_trained_guide_beats_plain_on_the_hard_room():
     print(f"trained {steps} steps on {used} samples (loss {loss:.3f}); relMSE at {spp} spp: "
           f"plain {np.mean(plain):.4f}, guided {np.mean(guided):.4f} ({np.mean(guided) / np.mean(plain):.3f}x)", flush=True)
     assert steps == 64 * 16
-    assert np.mean(guided) < 0.85 * np.mean(plain)
+    assert np.mean(guided) < 0.5 * np.mean(plain)
