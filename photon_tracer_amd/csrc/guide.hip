@@ -12,13 +12,30 @@ namespace guide {
 
 namespace {
 
-// Encodes the block's queries into s.a and runs the network; O holds the
-// 32 outputs of query threadIdx.x afterwards.
+// Runs the network on the block's queries (one wave) as the path loop's
+// inference does -- weights copied to LDS, activations in registers -- and
+// leaves query threadIdx.x's 32 outputs at o + threadIdx.x * OS.
+struct EvalLds {
+    LdsNet net;
+    float o[64 * OS];
+};
+
 __device__ __forceinline__ void eval_block(const GuideView& g, EvalLds& s, f3 x, f3 n) {
-    encode(g, x, n, s.a + threadIdx.x * XS);
+    load_net(g, s.net);
+    RegActs X;
+    encode_regs(g, x, n, X);
+    f16v O[2];
+    forward_regs(LdsWeights{s.net}, X, O);
+    const int c = lane_c(), kh = lane_kh();
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) s.o[(32 * nb + c) * OS + 8 * g4 + 4 * kh + t] = O[nb][4 * g4 + t];
+        }
+    }
     __syncthreads();
-    _Float16* X[5] = {s.a, s.b, s.a, s.b, s.a};
-    forward(g, X, nullptr, s.o);
 }
 
 }  // namespace

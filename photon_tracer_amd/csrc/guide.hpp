@@ -11,8 +11,10 @@
 // A wave handles 64 samples. Every product is computed transposed, Y^T =
 // W X^T, on v_mfma_f32_32x32x16_f16: the A operand is a row of W (8 inputs a
 // lane), the B operand is a sample's row of X (8 inputs a lane), and the
-// 32x32 result leaves each lane one sample's column of 16 output features,
-// written back to LDS as four 8-byte runs of the next layer's row.
+// 32x32 result leaves each lane one sample's column of 16 output features.
+// Training writes them back to LDS as four 8-byte runs of the next layer's
+// row (the backward pass needs every layer's activations); inference keeps
+// them in registers and reads the weights from an LDS copy (forward_regs).
 #pragma once
 
 #include "device.hpp"
@@ -56,28 +58,45 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
 
 // One-blob encoding: a value in [0,1] spread over `bins` Gaussian bumps of
-// width one bin centred at (i + 0.5) / bins.
+// width one bin centred at (i + 0.5) / bins, exp(-d^2 / 2) for d = v bins -
+// (i + 0.5), computed as 2^-(d k)^2 with k = sqrt(log2(e) / 2) folded into
+// both terms of d, on the hardware's 2^x (v_exp_f32). Its error, a few units
+// in the last place of FP32, is far below the FP16 rounding every input goes
+// through.
+constexpr float BLOB_K = 0.8493218002880191f;   // sqrt(log2(e) / 2)
+
+__device__ __forceinline__ float blob_scaled(float vk, float ck) {
+    const float d = vk - ck;
+    return __builtin_amdgcn_exp2f(-(d * d));
+}
 __device__ __forceinline__ float blob(float v, int bins, int i) {
-    const float d = v * static_cast<float>(bins) - (static_cast<float>(i) + 0.5f);
-    return expf(-0.5f * d * d);
+    return blob_scaled(v * (static_cast<float>(bins) * BLOB_K), (static_cast<float>(i) + 0.5f) * BLOB_K);
 }
 
-// The 64 inputs of a shading point: position normalised to the box, and the
-// normal's polar angle / pi and azimuth / (2 pi).
+// The five values a shading point is encoded from: its position normalised
+// to the box per axis, and the normal's polar angle / pi and azimuth / (2 pi).
+__device__ __forceinline__ void encode_scalars(const GuideView& g, f3 x, f3 n, float e[5]) {
+    e[0] = clamp01((x.x - g.lo.x) * g.inv_ext.x);
+    e[1] = clamp01((x.y - g.lo.y) * g.inv_ext.y);
+    e[2] = clamp01((x.z - g.lo.z) * g.inv_ext.z);
+    e[3] = clamp01(acosf(fminf(fmaxf(n.z, -1.0f), 1.0f)) * (1 / PI_F));
+    e[4] = clamp01((atan2f(n.y, n.x) + PI_F) * (1 / (2 * PI_F)));
+}
+
+// The 64 inputs of a shading point, as one row: 16 bins per position axis,
+// then 8 bins of the polar and 8 of the azimuth fraction.
 __device__ __forceinline__ void encode(const GuideView& g, f3 x, f3 n, _Float16* row) {
-    const float c[3] = {clamp01((x.x - g.lo.x) * g.inv_ext.x), clamp01((x.y - g.lo.y) * g.inv_ext.y),
-                        clamp01((x.z - g.lo.z) * g.inv_ext.z)};
+    float e[5];
+    encode_scalars(g, x, n, e);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) row[16 * a + i] = static_cast<_Float16>(blob(c[a], 16, i));
+        for (int i = 0; i < 16; ++i) row[16 * a + i] = static_cast<_Float16>(blob(e[a], 16, i));
     }
-    const float th = clamp01(acosf(fminf(fmaxf(n.z, -1.0f), 1.0f)) * (1 / PI_F));
-    const float ph = clamp01((atan2f(n.y, n.x) + PI_F) * (1 / (2 * PI_F)));
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        row[48 + i] = static_cast<_Float16>(blob(th, 8, i));
-        row[56 + i] = static_cast<_Float16>(blob(ph, 8, i));
+        row[48 + i] = static_cast<_Float16>(blob(e[3], 8, i));
+        row[56 + i] = static_cast<_Float16>(blob(e[4], 8, i));
     }
 }
 
@@ -338,12 +357,6 @@ __device__ __forceinline__ void output_store(const GuideView& g, const _Float16*
     }
 }
 
-// LDS of a forward-only wave: two ping-pong activation tiles and the outputs.
-struct EvalLds {
-    _Float16 a[64 * XS], b[64 * XS];
-    float o[64 * OS];
-};
-
 // The whole network on a wave's 64 encoded rows X[0]; hidden activations in
 // X[1..4] (and their transposes in XT[0..4] when XT is not null); outputs in O.
 __device__ __forceinline__ void forward(const GuideView& g, _Float16* const* X, _Float16* const* XT, float* O) {
@@ -429,6 +442,165 @@ __device__ __forceinline__ void layer_backward(const GuideView& g, int l, const 
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident forward pass (inference). A wave's 64 queries stay in
+// VGPRs from the encoding to the outputs: no LDS, no barrier. Lane (c, kh)
+// holds, for the queries c and 32 + c (nb = 0, 1), the 8 values of each
+// 16-wide k chunk kb that its half kh feeds the matrix unit. A layer's 32x32
+// result leaves lane (c, kh) the outputs 8 g4 + 4 kh + t (g4, t < 4) of row
+// block mb for its two queries; they stay where they are and become the next
+// layer's chunk kb = 2 mb + (g4 >> 1), element 4 (g4 & 1) + t. The product
+// sums over every k of a chunk whatever their order, so the next layer's
+// weight operand just takes its columns in the same order: element 4 s + t of
+// chunk kb is column 16 kb + 8 s + 4 kh + t (two 8-byte loads).
+// ---------------------------------------------------------------------------
+typedef h8 RegActs[2][4];
+
+// The encoded inputs in that layout (the first layer's chunks hold columns in
+// their natural order: chunk kb, half kh = inputs 16 kb + 8 kh + j). Same
+// arithmetic as encode(), so the same FP16 inputs.
+__device__ __forceinline__ void encode_regs(const GuideView& g, f3 x, f3 n, RegActs& X) {
+    const int kh = lane_kh();
+    float e[5], o[5];
+    encode_scalars(g, x, n, e);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) o[v] = __shfl_xor(e[v], 32, 64);   // the query of the other half's lane
+    float c16[8], c8[8];   // bin centres of this lane's half, scaled
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        c16[j] = (static_cast<float>(8 * kh + j) + 0.5f) * BLOB_K;
+        c8[j] = (static_cast<float>(j) + 0.5f) * BLOB_K;
+    }
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+        float q[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = nb == kh ? e[v] : o[v];
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+            const float vk = q[kb] * (16.0f * BLOB_K);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) X[nb][kb][j] = static_cast<_Float16>(blob_scaled(vk, c16[j]));
+        }
+        const float vk = (kh ? q[4] : q[3]) * (8.0f * BLOB_K);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) X[nb][3][j] = static_cast<_Float16>(blob_scaled(vk, c8[j]));
+    }
+}
+
+// ReLU of 8 values rounded to FP16: the conversion (round to nearest even),
+// then a 16-bit integer max with 0 -- a negative value or -0 has its sign bit
+// set and reads as a negative integer, 0 and positive values keep their bits.
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef float f8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ h8 relu_h8(f8 v) {
+    const s8 bits = __builtin_bit_cast(s8, __builtin_convertvector(v, h8));
+    return __builtin_bit_cast(h8, __builtin_elementwise_max(bits, s8(0)));
+}
+
+// The network's FP16 weights and FP32 biases copied to LDS once per block:
+// rows padded to WROW halves (a wave's 16-byte row reads spread over the
+// banks), rows of all layers one after another (layer l's row j at
+// 64 l + j), and the columns of layers 1..4 stored in the register order,
+// so every weight operand is one 16-byte read at a lane base plus an
+// immediate offset.
+constexpr int WROW = IN + 8;
+constexpr int NET_ROWS = 4 * HID + OUT;
+struct LdsNet {
+    _Float16 w[NET_ROWS * WROW];
+    float b[NET_ROWS];
+};
+
+// Fills s from the mirrors; all threads of the block take part (then a
+// barrier). Column 16 kb + 8 s + 4 kh + t of a row of layers 1..4 goes to
+// 16 kb + 8 kh + 4 s + t.
+__device__ __forceinline__ void load_net(const GuideView& g, LdsNet& s) {
+    for (uint32_t idx = threadIdx.x; idx < static_cast<uint32_t>(NET_ROWS) * (IN / 8); idx += blockDim.x) {
+        const uint32_t r = idx >> 3, k8 = idx & 7u;
+        const int l = r < 4u * HID ? static_cast<int>(r / HID) : 4;
+        const uint32_t row = r - static_cast<uint32_t>(l) * HID;
+        const h8 v = *reinterpret_cast<const h8*>(g.w16 + w_off(l) + (row * IN + 8u * k8));
+        _Float16* d = s.w + r * WROW;
+        if (l == 0) {
+            *reinterpret_cast<h8*>(d + 8u * k8) = v;
+        } else {
+            const uint32_t kb = k8 >> 1, sb = k8 & 1u;
+            *reinterpret_cast<h4*>(d + 16u * kb + 4u * sb) = h4{v[0], v[1], v[2], v[3]};
+            *reinterpret_cast<h4*>(d + 16u * kb + 8u + 4u * sb) = h4{v[4], v[5], v[6], v[7]};
+        }
+    }
+    for (uint32_t r = threadIdx.x; r < static_cast<uint32_t>(NET_ROWS); r += blockDim.x) {
+        const int l = r < 4u * HID ? static_cast<int>(r / HID) : 4;
+        s.b[r] = g.p32[b_off(l) + (r - static_cast<uint32_t>(l) * HID)];
+    }
+    __syncthreads();
+}
+
+struct LdsWeights {
+    const LdsNet& s;
+    __device__ __forceinline__ h8 a(int l, int mb, int kb) const {
+        const uint32_t r = static_cast<uint32_t>(l * HID + mb * 32 + lane_c());
+        return *reinterpret_cast<const h8*>(s.w + (r * WROW + 16u * kb + 8u * lane_kh()));
+    }
+    __device__ __forceinline__ float4 bias(int l, int mb, int g4) const {
+        return *reinterpret_cast<const float4*>(s.b + (static_cast<uint32_t>(l * HID + mb * 32) + 4u * lane_kh() + 8u * g4));
+    }
+};
+
+// Row block mb of layer l for both query blocks: acc[nb] = W X^T + b.
+template <class Src>
+__device__ __forceinline__ void block_product(const Src& src, int l, const RegActs& X, int mb, f16v (&acc)[2]) {
+    h8 a[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) a[kb] = src.a(l, mb, kb);
+#pragma unroll
+    for (int g4 = 0; g4 < 4; ++g4) {
+        const float4 bb = src.bias(l, mb, g4);
+        acc[0][4 * g4] = bb.x;
+        acc[0][4 * g4 + 1] = bb.y;
+        acc[0][4 * g4 + 2] = bb.z;
+        acc[0][4 * g4 + 3] = bb.w;
+    }
+    acc[1] = acc[0];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], X[nb][kb], acc[nb], 0, 0, 0);
+    }
+}
+
+// The whole network on X; O[nb][4 g4 + t] is output 8 g4 + 4 kh + t of
+// query 32 nb + c (FP32, before any rounding). Src supplies the weight
+// operands and biases (LdsWeights).
+template <class Src>
+__device__ __forceinline__ void forward_regs(const Src& src, RegActs& X, f16v (&O)[2]) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        RegActs Y;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+            f16v acc[2];
+            block_product(src, l, X, mb, acc);
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    f8 v;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v[j] = acc[nb][8 * h + j];
+                    Y[nb][2 * mb + h] = relu_h8(v);
+                }
+            }
+        }
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) X[nb][kb] = Y[nb][kb];
+        }
+    }
+    block_product(src, 4, X, 0, O);
 }
 
 }  // namespace guide

@@ -23,6 +23,11 @@ namespace {
 
 constexpr int SHADE_BLOCK = 1024;
 constexpr int SHADE_WAVES = SHADE_BLOCK / 64;
+// The guide's inference: blocks of eight independent waves sharing one LDS
+// copy of the network (42 KB), 2 blocks per CU: 4 waves per SIMD, what the
+// kernel's 128 registers allow.
+constexpr int INFER_BLOCK = 512;
+constexpr int INFER_BLOCKS_PER_CU = 2;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
 
@@ -586,19 +591,26 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
 
 // The guide's outputs for the shade stage of bounce b: for every queued hit
 // on a diffuse (Lambertian) surface, the network at its point and normal,
-// 32 FP16 values at gout[slot]. A block is one wave of 64 slots (18 KB of
-// LDS: two activation tiles; the output layer goes from the matrix unit's
-// registers to HBM); waves with no diffuse hit skip the network.
+// 32 FP16 values at gout[slot]. The block first copies the network's weights
+// into LDS; then each wave takes 64 slots at a time and keeps them in
+// registers from the encoding to the outputs (no barrier after the copy);
+// waves with no diffuse hit skip the network. Weights read from the caches
+// instead of LDS made the kernel 2.5x slower: their per-layer addresses,
+// hoisted out of the loop by the compiler, cost half the registers.
 template <bool PRIMARY>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_guide_infer(FrameParams fp, int b) {
+__global__ __launch_bounds__(INFER_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_guide_infer(FrameParams fp, int b) {
     if (fp.frame->tail_from <= static_cast<uint32_t>(b)) return;
-    __shared__ _Float16 xa[64 * guide::XS], xb[64 * guide::XS];
     const uint32_t n = PRIMARY ? fp.P : fp.ctrl[b].n_rays;
     FrameConst fc;
     if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1;
-    for (uint32_t base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
-        const uint32_t q = base + threadIdx.x;
+    __shared__ guide::LdsNet net;
+    guide::load_net(fp.gv, net);
+    const int c = guide::lane_c(), kh = guide::lane_kh();
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * INFER_BLOCK;
+    for (uint32_t base = blockIdx.x * INFER_BLOCK + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t q = base + lane;
         bool diffuse = false;
         f3 pnt = mk(0, 0, 0), nrm = mk(0, 0, 1);
         if (q < n) {
@@ -621,18 +633,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         }
         const unsigned long long rows = __ballot(diffuse);
         if (rows == 0) continue;   // the whole wave agrees
-        guide::encode(fp.gv, pnt, nrm, xa + threadIdx.x * guide::XS);
-        __syncthreads();
-        guide::layer_forward<guide::HID>(fp.gv, 0, xa, xb, nullptr, nullptr);
-        __syncthreads();
-        guide::layer_forward<guide::HID>(fp.gv, 1, xb, xa, nullptr, nullptr);
-        __syncthreads();
-        guide::layer_forward<guide::HID>(fp.gv, 2, xa, xb, nullptr, nullptr);
-        __syncthreads();
-        guide::layer_forward<guide::HID>(fp.gv, 3, xb, xa, nullptr, nullptr);
-        __syncthreads();
-        guide::output_store(fp.gv, xa, rows, fp.gout + static_cast<size_t>(base) * guide::OUT);
-        __syncthreads();   // the tiles are reused by the next slots
+        guide::RegActs X;
+        guide::encode_regs(fp.gv, pnt, nrm, X);
+        guide::f16v O[2];
+        guide::forward_regs(guide::LdsWeights{net}, X, O);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int i = 32 * nb + c;
+            if ((rows >> i) & 1ull) {
+                _Float16* out = fp.gout + static_cast<size_t>(base + i) * guide::OUT + 4 * kh;
+#pragma unroll
+                for (int g4 = 0; g4 < 4; ++g4) {
+                    guide::h4 hv;
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) hv[t] = static_cast<_Float16>(O[nb][4 * g4 + t]);
+                    *reinterpret_cast<guide::h4*>(out + 8 * g4) = hv;
+                }
+            }
+        }
     }
 }
 
@@ -994,9 +1012,9 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
         }
         if (ev) (void)hipEventRecord(ev[2 * b + 1], stream);
         if (fp.guide_mode && b < fp.guide_bounces) {
-            const int infer_grid = (shade_grid / 2) * 8;   // 8 one-wave blocks per CU (18 KB of LDS each)
-            if (b == 0) k_guide_infer<true><<<infer_grid, 64, 0, stream>>>(fp, b);
-            else k_guide_infer<false><<<infer_grid, 64, 0, stream>>>(fp, b);
+            const int infer_grid = (shade_grid / 2) * INFER_BLOCKS_PER_CU;
+            if (b == 0) k_guide_infer<true><<<infer_grid, INFER_BLOCK, 0, stream>>>(fp, b);
+            else k_guide_infer<false><<<infer_grid, INFER_BLOCK, 0, stream>>>(fp, b);
         }
         if (b == 0) {
             if (guided) k_shade<true, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
