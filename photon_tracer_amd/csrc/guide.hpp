@@ -509,8 +509,8 @@ __device__ __forceinline__ h8 relu_h8(f8 v) {
 constexpr int WROW = IN + 8;
 constexpr int NET_ROWS = 4 * HID + OUT;
 struct LdsNet {
-    _Float16 w[NET_ROWS * WROW];
-    float b[NET_ROWS];
+    alignas(16) _Float16 w[NET_ROWS * WROW];
+    alignas(16) float b[NET_ROWS];
 };
 
 // Fills s from the mirrors; all threads of the block take part (then a
@@ -540,12 +540,18 @@ __device__ __forceinline__ void load_net(const GuideView& g, LdsNet& s) {
 
 struct LdsWeights {
     const LdsNet& s;
+    // Every row starts on a 16-byte boundary (WROW halves = 144 bytes), and
+    // so does every read: one ds_read_b128 at the lane's base plus an
+    // immediate offset.
     __device__ __forceinline__ h8 a(int l, int mb, int kb) const {
         const uint32_t r = static_cast<uint32_t>(l * HID + mb * 32 + lane_c());
-        return *reinterpret_cast<const h8*>(s.w + (r * WROW + 16u * kb + 8u * lane_kh()));
+        const h8* p = reinterpret_cast<const h8*>(s.w + (r * WROW + 16u * kb + 8u * lane_kh()));
+        return *static_cast<const h8*>(__builtin_assume_aligned(p, 16));
     }
     __device__ __forceinline__ float4 bias(int l, int mb, int g4) const {
-        return *reinterpret_cast<const float4*>(s.b + (static_cast<uint32_t>(l * HID + mb * 32) + 4u * lane_kh() + 8u * g4));
+        const float4* p = reinterpret_cast<const float4*>(
+            s.b + (static_cast<uint32_t>(l * HID + mb * 32) + 4u * lane_kh() + 8u * g4));
+        return *static_cast<const float4*>(__builtin_assume_aligned(p, 16));
     }
 };
 
