@@ -178,6 +178,25 @@ def batch_leg(pt, n_jobs, steps, tune=True, graph=True):
     return out
 
 
+# The guide network's FLOPs per evaluated hit: four 64x64 layers and a 64x32
+# output layer, two FLOPs a multiply-add (biases not counted).
+MLP_FLOPS = 2 * (4 * 64 * 64 + 64 * 32)
+MFMA_F16_PEAK_TFLOPS = 2500.0   # MI355X dense FP16 matrix peak
+
+
+def network_roofline(s, W, H, spp):
+    """The guide's inference kernel against the FP16 MFMA peak: one counting
+    render (diffuse hits evaluated) and one timing render (HIP events around
+    each inference launch) of the same guided frame."""
+    _, _, cst = s.render(W, H, spp, seed=3, guide=True, count=True)
+    _, _, tst = s.render(W, H, spp, seed=3, guide=True, timing=True)
+    achieved = cst.guide_queries * MLP_FLOPS / (tst.guide_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(achieved, 1), "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / MFMA_F16_PEAK_TFLOPS, 4), "kernel": "k_guide_infer (all guided bounces)",
+            "workload": f"{W}x{H}, {spp} spp guided", "queries": cst.guide_queries,
+            "flops_per_query": MLP_FLOPS, "ms": round(tst.guide_ms, 3), "launches": 10 * tst.batches}
+
+
 def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2, 3, 4)):
     """Equal-time relMSE of the guided and plain tracers on the hard-lighting
     room at 1080p. The reference is ref_spp plain samples in two independent
@@ -236,6 +255,9 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
             e_guided = rel_mse(guided)
             guided_eq, _, _ = s.render(W, H, spp_plain, seed=41, guide=True)
             e_guided_eq = rel_mse(guided_eq)
+            if not runs:
+                net = network_roofline(s, W, H, 16)
+                log(f"guided: network roofline {json.dumps(net)}")
             s.set_guide(None)
             g.close()
             # relMSE falls as 1/spp: the guided time to reach the plain tracer's relMSE.
@@ -263,6 +285,7 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
         "time_to_plain_relmse_ratio": med("time_to_plain_relmse_ratio"),
         "target_time_ratio": 0.5,
         "statistic": f"median over guide seeds {list(guide_seeds)}",
+        "network_roofline": net,
     }
     log(f"guided: {json.dumps({k: v for k, v in out.items() if k != 'guided_runs'})}")
     return out

@@ -181,6 +181,10 @@ typedef struct {
                                   and the number of queries those launches traced */
     int32_t batches;           /* batches the samples were split into */
     int32_t samples_per_batch; /* samples per pixel in each batch */
+    double guide_ms;           /* guided renders with PT_RENDER_TIMING: time in the guide's inference
+                                  launches (the network at every guided bounce) */
+    uint64_t guide_queries;    /* guided renders with PT_RENDER_COUNT: diffuse hits the guide's
+                                  inference evaluated the network for */
 } pt_render_stats;
 
 /* rgb: 3 floats a pixel, top row first, for the rows rendered (the band's top

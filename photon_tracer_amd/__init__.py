@@ -73,6 +73,7 @@ class _RenderStats(ctypes.Structure):
         ("rays", ctypes.c_uint64), ("primary_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
         ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
         ("visits", ctypes.c_uint64 * 7), ("batches", ctypes.c_int32), ("samples_per_batch", ctypes.c_int32),
+        ("guide_ms", ctypes.c_double), ("guide_queries", ctypes.c_uint64),
     ]
 
 
@@ -263,6 +264,8 @@ class RenderStats:
     visits: tuple = (0, 0, 0, 0, 0, 0, 0)
     batches: int = 0
     samples_per_batch: int = 0
+    guide_ms: float = 0.0
+    guide_queries: int = 0
 
     @property
     def mrays_per_s(self) -> float:
@@ -364,7 +367,7 @@ class Scene:
             _check(library().pt_render(self._h, ctypes.byref(p), rgb.ctypes.data, ax.ctypes.data if aux else None,
                                        ctypes.byref(st)))
         stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
-                            tuple(st.visits), st.batches, st.samples_per_batch)
+                            tuple(st.visits), st.batches, st.samples_per_batch, st.guide_ms, st.guide_queries)
         return rgb, ax, stats
 
 

@@ -523,12 +523,14 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         HIPC(hipMemcpyAsync(B.frame.p, &fc, sizeof fc, hipMemcpyHostToDevice, s->stream));
         HIPC(ptd::launch_clear_frame(fp, s->stream));
         HIPC(hipEventRecord(e0, s->stream));
-        double extend_ms = 0, shadow_ms = 0;
+        double extend_ms = 0, shadow_ms = 0, guide_ms = 0;
         const bool timing = stats && (P.flags & PT_RENDER_TIMING);
         if (timing) {
-            // Eager, with an event pair around every traversal launch.
-            std::vector<hipEvent_t> ev(2 * P.max_bounces);
+            // Eager, with an event pair around every traversal launch and,
+            // in guided renders, every inference launch (pairs 2 depth ..).
+            std::vector<hipEvent_t> ev(4 * P.max_bounces);
             for (auto& e : ev) HIPC(hipEventCreate(&e));
+            const int guided_bounces = fp.guide_mode ? std::min(P.max_bounces, fp.guide_bounces) : 0;
             for (int b = 0; b < batches; ++b) {
                 HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, ev.data()));
                 HIPC(hipStreamSynchronize(s->stream));
@@ -536,6 +538,11 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
                     float ms = 0;
                     HIPC(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
                     extend_ms += ms;
+                }
+                for (int k = 0; k < guided_bounces; ++k) {
+                    float ms = 0;
+                    HIPC(hipEventElapsedTime(&ms, ev[2 * (P.max_bounces + k)], ev[2 * (P.max_bounces + k) + 1]));
+                    guide_ms += ms;
                 }
             }
             for (auto& e : ev) (void)hipEventDestroy(e);
@@ -604,11 +611,13 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             stats->render_ms = render_ms;
             stats->extend_ms = extend_ms;
             stats->shadow_ms = shadow_ms;
-            unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
-            if (counting) HIPC(hipMemcpy(v, s->counts.p, 56, hipMemcpyDeviceToHost));
+            unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (counting) HIPC(hipMemcpy(v, s->counts.p, sizeof v, hipMemcpyDeviceToHost));
             for (int k = 0; k < 7; ++k) stats->visits[k] = v[k];
             stats->batches = batches;
             stats->samples_per_batch = spb;
+            stats->guide_ms = guide_ms;
+            stats->guide_queries = v[7];
         }
     });
 }

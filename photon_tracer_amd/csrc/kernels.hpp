@@ -60,7 +60,8 @@ struct FrameParams {
     FrameConst* frame;        // seed, first sample, batch number
     unsigned long long* totals;  // [0] closest-hit rays, [1] shadow rays, [2] primary rays
     uint2* spill;             // traversal stack spill area
-    unsigned long long* visits;  // null, or seven totals the traversal stage counts into (Visits)
+    unsigned long long* visits;  // null, or seven totals the traversal stage counts into (Visits), then
+                                 // the diffuse hits the guide's inference evaluated
     uint32_t tail_threshold;  // queues below this many rays go to the tail launch (0: never)
     // The learned guide: 0 off, 1 diffuse bounces sample it (one-sample MIS
     // with the cosine lobe), 2 they also record training vertices.

@@ -633,6 +633,7 @@ __global__ __launch_bounds__(INFER_BLOCK) __attribute__((amdgpu_waves_per_eu(4))
         }
         const unsigned long long rows = __ballot(diffuse);
         if (rows == 0) continue;   // the whole wave agrees
+        if (fp.visits && lane == 0) atomicAdd(fp.visits + 7, static_cast<unsigned long long>(__popcll(rows)));
         guide::RegActs X;
         guide::encode_regs(fp.gv, pnt, nrm, X);
         guide::f16v O[2];
@@ -1013,8 +1014,10 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
         if (ev) (void)hipEventRecord(ev[2 * b + 1], stream);
         if (fp.guide_mode && b < fp.guide_bounces) {
             const int infer_grid = (shade_grid / 2) * INFER_BLOCKS_PER_CU;
+            if (ev) (void)hipEventRecord(ev[2 * (fp.depth + b)], stream);
             if (b == 0) k_guide_infer<true><<<infer_grid, INFER_BLOCK, 0, stream>>>(fp, b);
             else k_guide_infer<false><<<infer_grid, INFER_BLOCK, 0, stream>>>(fp, b);
+            if (ev) (void)hipEventRecord(ev[2 * (fp.depth + b) + 1], stream);
         }
         if (b == 0) {
             if (guided) k_shade<true, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
