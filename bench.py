@@ -291,6 +291,36 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
     return out
 
 
+def configs_leg(pt, cpu_seconds, frames=3):
+    """The other plain workloads, each against the reference renderer on the
+    host cores: the sphere field (640x360, 16 spp, sky on) and the
+    two-level instancing scene (three meshes of 50k, 200k and 500k
+    triangles, 4096 instances; 1920x1080, 32 spp). GPU rate: rays over the
+    device time of `frames` renders after one warmup, image kept in HBM."""
+    out = {}
+    for name, cfg, W, H, spp in (("spheres", 0, 640, 360, 16), ("instances", 2, 1920, 1080, 32)):
+        t0 = time.perf_counter()
+        ptsd = pt.generate(cfg, 1, 1.0)
+        ptgp = pt.pack(ptsd)
+        log(f"{name}: generated and packed in {time.perf_counter() - t0:.1f} s")
+        image = pt.DeviceBuffer(12 * W * H)
+        with pt.Scene(ptgp) as s:
+            s.render(W, H, spp, seed=1, device_out=image)
+            rays, ms = 0, 0.0
+            for k in range(frames):
+                _, _, st = s.render(W, H, spp, seed=2 + k, device_out=image)
+                rays += st.rays
+                ms += st.render_ms
+        image.free()
+        gpu = rays / (ms * 1e-3) / 1e6
+        cpu = cpu_baseline(ptsd, W, H, spp, cpu_seconds)
+        out[name] = {"workload": f"{W}x{H}, {spp} spp, 10 bounces, NEE", "mrays_per_s": round(gpu, 1),
+                     "ms_per_frame": round(ms / frames, 2), "rays_per_frame": rays // frames, "cpu_baseline": cpu,
+                     "gpu_over_cpu": round(gpu / cpu["value"], 1)}
+        log(f"{name}: {json.dumps(out[name])}")
+    return out
+
+
 def guided_kernels(pt):
     """Plain and guided 16-spp 1080p renders of the room, twice each, after
     four training batches: the kernels a profiler should see per render."""
@@ -332,7 +362,7 @@ def main():
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--guided-plain-spp", type=int, default=1024,
                     help="the equal-time budget: the plain render at this many spp")
-    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels"], default=None,
+    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels", "configs"], default=None,
                     help="run one leg alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also copy MIB MiB device to device (a kernel of known traffic, to scale HBM counters)")
@@ -348,7 +378,8 @@ def main():
         leg = {"batch": lambda: batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
                                           not args.batch_eager),
                "guided": lambda: guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp),
-               "guided-kernels": lambda: guided_kernels(pt)}[args.only]
+               "guided-kernels": lambda: guided_kernels(pt),
+               "configs": lambda: configs_leg(pt, args.cpu_seconds)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
 

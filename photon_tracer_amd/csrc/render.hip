@@ -176,7 +176,11 @@ struct refill_threshold {
     static constexpr uint32_t lanes = PRIMARY ? 56 : 48;
 };
 
-template <int S, bool COUNT, bool PRIMARY, class Stack>
+// Scenes with instances walk with lane refill too (the instance tree's steps
+// in the same walk); false keeps the per-wave fetch of 64 queries for them.
+constexpr bool INST_REFILL = true;
+
+template <int S, bool COUNT, bool PRIMARY, bool INST, class Stack>
 __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl* c, uint32_t n_shadow, uint32_t n,
                                               Stack& st, Visits& vis) {
     const float4* ro = fp.ro[b & 1];
@@ -189,7 +193,7 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
     bool drained = false;              // the queue has no more items
     bool live = false, any = false;
     uint32_t item = 0;
-    Walk w;
+    Walk<INST> w;
     while (true) {
         const unsigned long long idle = __ballot(!live);
         const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
@@ -220,7 +224,7 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
                         tmin = a.w;
                         tmax = bb.w;
                     }
-                    live = !walk_begin<COUNT>(fp.sc, w, o, d, tmin, tmax, any, st, &vis);
+                    live = !walk_begin<COUNT, INST>(fp.sc, w, o, d, tmin, tmax, any, st, &vis);
                     if (!live) {   // ended at once: only an any-hit query on a large sphere
                         item |= 0x80000000u;
                     }
@@ -229,7 +233,7 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
             pool = min(pool + n_idle, pool_end);
         }
         bool done = (item & 0x80000000u) != 0;
-        if (live && walk_step<COUNT>(fp.sc, w, any, st, &vis)) {
+        if (live && walk_step<COUNT, INST>(fp.sc, w, any, st, &vis)) {
             live = false;
             done = true;
         }
@@ -246,7 +250,8 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
                 h.v = w.hv;
                 h.kind = w.kind;
                 h.prim = w.prim;
-                h.inst = NONE;
+                if constexpr (INST) h.inst = w.inst;
+                else h.inst = NONE;
                 fp.hits[item - n_shadow] = h;
             }
             item = 0;
@@ -266,8 +271,8 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void
     const uint32_t n_rays = PRIMARY ? fp.P : c->n_rays;
     const uint32_t n = n_shadow + n_rays;
     if (PRIMARY && blockIdx.x == 0 && threadIdx.x == 0) c->n_rays = fp.P;
-    if constexpr (!INST) {
-        extend_refill<S, COUNT, PRIMARY>(fp, b, c, n_shadow, n, st, vis);
+    if constexpr (!INST || INST_REFILL) {
+        extend_refill<S, COUNT, PRIMARY, INST>(fp, b, c, n_shadow, n, st, vis);
         if (COUNT) add_visits(fp.visits, vis);
         return;
     }

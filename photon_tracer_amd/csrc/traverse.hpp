@@ -292,11 +292,24 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
     return kind != 0;
 }
 
-// One ray's walk of the flat tree, advanced one stack entry per step, so a
-// lane whose ray has ended can start another while the rest of its wave is
-// still walking. A ray goes through exactly the steps trace_ray takes for it
-// (scenes without instances), with the same results.
-struct Walk {
+// One ray's walk, advanced one stack entry per step, so a lane whose ray has
+// ended can start another while the rest of its wave is still walking. A ray
+// goes through exactly the steps trace_ray takes for it, with the same
+// results: the large spheres, the flat tree, then (INST) the instance tree,
+// whose leaves enter each instance's mesh tree in object space.
+template <bool INST>
+struct WalkInst {};
+template <>
+struct WalkInst<true> {
+    f3 lo, ld, linv;        // the ray in the entered instance's object space
+    uint32_t li, lend;      // the instance-tree leaf's slots still to enter: [li, lend)
+    uint32_t cur, inst;     // the instance entered; the instance of the nearest hit (NONE: none)
+    int blas_base;          // stack depth when the instance was entered
+    bool tlas, in_blas;     // walking the instance tree; inside an instance's mesh tree
+};
+
+template <bool INST = false>
+struct Walk : WalkInst<INST> {
     f3 o, d, inv;
     float tmin, tmax, hu, hv;
     uint32_t kind, prim;
@@ -307,8 +320,8 @@ struct Walk {
 // Starts a walk: the large spheres in double, then the flat tree's root.
 // Returns true if the query has already ended (an any-hit query that hit a
 // large sphere).
-template <bool COUNT, class Stack>
-__device__ __forceinline__ bool walk_begin(const SceneView& sc, Walk& w, f3 o, f3 d, float tmin, float tmax,
+template <bool COUNT, bool INST = false, class Stack>
+__device__ __forceinline__ bool walk_begin(const SceneView& sc, Walk<INST>& w, f3 o, f3 d, float tmin, float tmax,
                                            bool any, Stack& st, Visits* vis) {
     if (COUNT) ++vis->queries;
     w.o = o;
@@ -317,6 +330,10 @@ __device__ __forceinline__ bool walk_begin(const SceneView& sc, Walk& w, f3 o, f
     w.tmax = tmax;
     w.hu = w.hv = 0;
     w.kind = w.prim = 0;
+    if constexpr (INST) {
+        w.inst = NONE;
+        w.tlas = w.in_blas = false;
+    }
     for (uint32_t k = 0; k < sc.n_large; ++k) {
         if (COUNT) ++vis->large;
         const uint32_t s = sc.large[k];
@@ -333,13 +350,53 @@ __device__ __forceinline__ bool walk_begin(const SceneView& sc, Walk& w, f3 o, f
     return false;
 }
 
-// One stack entry of the walk: a node's slab test, or a leaf's primitives.
-// Returns true when the walk has ended (the stack is empty, or an any-hit
-// query found a hit); kind != 0 then says something was hit.
+// Enters instance k of an instance-tree leaf: the ray to its object space
+// (the same t there), its mesh tree's root next (trace_ray's `enter`).
 template <bool COUNT, class Stack>
-__device__ __forceinline__ bool walk_step(const SceneView& sc, Walk& w, bool any, Stack& st, Visits* vis) {
+__device__ __forceinline__ void walk_enter(const SceneView& sc, Walk<true>& w, uint32_t k, Stack& st, Visits* vis) {
+    if (COUNT) ++vis->instances;
+    w.cur = k;
+    const float4 r0 = sc.inst[7 * k + 3], r1 = sc.inst[7 * k + 4], r2 = sc.inst[7 * k + 5];
+    const uint32_t root = __float_as_uint(sc.inst[7 * k + 6].x);
+    const f3 o = w.o, d = w.d;
+    w.lo = mk(r0.x * o.x + r0.y * o.y + r0.z * o.z + r0.w, r1.x * o.x + r1.y * o.y + r1.z * o.z + r1.w,
+              r2.x * o.x + r2.y * o.y + r2.z * o.z + r2.w);
+    w.ld = mk(r0.x * d.x + r0.y * d.y + r0.z * d.z, r1.x * d.x + r1.y * d.y + r1.z * d.z,
+              r2.x * d.x + r2.y * d.y + r2.z * d.z);
+    w.linv = safe_rcp3(w.ld);
+    w.blas_base = st.top;
+    w.in_blas = true;
+    w.e = make_uint2(root, __float_as_uint(w.tmin));
+    w.have = root != NONE;
+}
+
+// One stack entry of the walk: a node's slab test, or a leaf's primitives
+// (INST: or the instance tree's bookkeeping: the next instance of a leaf, the
+// switch from the flat tree to the instance tree). Returns true when the walk
+// has ended (every tree walked, or an any-hit query found a hit); kind != 0
+// then says something was hit.
+template <bool COUNT, bool INST = false, class Stack>
+__device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bool any, Stack& st, Visits* vis) {
     if (!w.have) {
-        if (st.top == 0) return true;
+        if constexpr (INST) {
+            if (w.in_blas && st.top == w.blas_base) {   // the entered instance's mesh tree is done
+                if (++w.li < w.lend) {
+                    walk_enter<COUNT>(sc, w, sc.order[w.li], st, vis);
+                    return false;
+                }
+                w.in_blas = false;
+            }
+            if (st.top == 0) {
+                if (w.tlas || sc.inst_root == NONE) return true;
+                w.tlas = true;   // the flat tree is done: the instance tree next
+                st.reset();
+                w.e = make_uint2(sc.inst_root, __float_as_uint(w.tmin));
+                w.have = true;
+                return false;
+            }
+        } else {
+            if (st.top == 0) return true;
+        }
         w.e = st.pop();
     }
     w.have = false;
@@ -347,8 +404,36 @@ __device__ __forceinline__ bool walk_step(const SceneView& sc, Walk& w, bool any
     const uint32_t first = w.e.x & FIRST_MASK, count = w.e.x >> COUNT_SHIFT;
     if (count == 0) {
         if (COUNT) ++vis->nodes;
+        if constexpr (INST) {
+            if (w.in_blas) {
+                w.have = visit_node(sc.nodes, first, w.lo, w.linv, w.tmin, w.tmax, st, w.e);
+                return false;
+            }
+        }
         w.have = visit_node(sc.nodes, first, w.o, w.inv, w.tmin, w.tmax, st, w.e);
         return false;
+    }
+    if constexpr (INST) {
+        if (w.tlas) {
+            if (COUNT) vis->leafrefs += count;
+            if (!w.in_blas) {   // an instance-tree leaf: enter its instances in turn
+                w.li = first;
+                w.lend = first + count;
+                walk_enter<COUNT>(sc, w, sc.order[first], st, vis);
+                return false;
+            }
+            for (uint32_t i = first; i < first + count; ++i) {   // a mesh leaf, in object space
+                if (COUNT) ++vis->tris;
+                const float4 r0 = sc.leaf[3 * i], r1 = sc.leaf[3 * i + 1], r2 = sc.leaf[3 * i + 2];
+                if (tri_hit(xyz(r0), xyz(r1), xyz(r2), w.lo, w.ld, w.tmin, w.tmax, w.hu, w.hv)) {
+                    w.kind = 3;
+                    w.prim = __float_as_uint(r0.w);
+                    w.inst = w.cur;
+                    if (any) return true;
+                }
+            }
+            return false;
+        }
     }
     for (uint32_t i = first; i < first + count; ++i) {
         const float4 r0 = sc.leaf[3 * i], r1 = sc.leaf[3 * i + 1];
@@ -368,6 +453,7 @@ __device__ __forceinline__ bool walk_step(const SceneView& sc, Walk& w, bool any
         }
         if (hit) {
             w.prim = __float_as_uint(r0.w);
+            if constexpr (INST) w.inst = NONE;
             if (any) return true;
         }
     }
