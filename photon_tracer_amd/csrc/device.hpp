@@ -162,15 +162,19 @@ __device__ __forceinline__ uint32_t lk_permute(uint32_t x, uint32_t seed) {
 
 // The second Sobol dimension of index i, bit-reversed: XOR of the direction
 // numbers v_k (v_0 = 1 << 31, v_{k+1} = v_k ^ (v_k >> 1)) of i's set bits,
-// then reversed (sampler.hpp:63-94). Computed bit by bit rather than from
-// byte tables; only set bits cost work.
-__device__ __forceinline__ uint32_t sobol_1_reversed(uint32_t i) {
-    uint32_t r = 0, v = 1u << 31;
-    for (int k = 0; k < 32; ++k) {
-        if (i & (1u << k)) r ^= v;
-        v ^= v >> 1;
-    }
-    return __builtin_bitreverse32(r);
+// then reversed (sampler.hpp:63-94). v_k is row k of Pascal's triangle mod 2,
+// high bit first, so bit j of the reversed result is the XOR of i's bits k
+// with C(k, j) odd, which by Lucas' theorem are the k whose binary digits
+// include j's: a superset-XOR transform over the 5-bit bit positions, one
+// shift, mask and XOR per position bit (both sides are linear over GF(2) and
+// agree on the 32 unit vectors).
+__device__ __forceinline__ uint32_t sobol_1_reversed(uint32_t x) {
+    x ^= (x >> 1) & 0x55555555u;
+    x ^= (x >> 2) & 0x33333333u;
+    x ^= (x >> 4) & 0x0f0f0f0fu;
+    x ^= (x >> 8) & 0x00ff00ffu;
+    x ^= (x >> 16) & 0x0000ffffu;
+    return x;
 }
 
 // begin_sample (sampler.hpp:98-104)
