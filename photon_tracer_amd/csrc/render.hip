@@ -160,20 +160,22 @@ __device__ __forceinline__ void light_sample_arrives(const FrameParams& fp, uint
 // persistent grid from one counter, shadow rays first. PRIMARY: bounce 0,
 // whose rays are the batch's camera rays, made here; it has no shadow rays.
 //
-// Scenes without instances walk with lane refill: each lane advances its own
-// query one stack entry at a time, and once at least REFILL lanes of a wave
+// The traversal stage walks with lane refill: each lane advances its own
+// query one stack entry at a time, and once at least `lanes` lanes of a wave
 // have finished theirs, those lanes take the next items from the wave's run
 // of the queue (one atomic per run of 64 * fetch_groups items). A wave then
 // stays busy with most of its lanes, instead of waiting for its longest walk
 // before taking 64 new ones. Starting a query costs the whole wave the setup
 // code (a camera ray: the QMC jitter and the camera; a queued ray: its loads
-// and three divisions), so lanes wait until most of the wave is idle: 56 of
-// 64 for camera rays, 48 for queued ones (measured on the 1M-triangle
-// icosphere at 1080p; the speed is flat from 40 to 60). Each query still
-// goes through the same steps, so the hits are the same.
+// and three divisions), so lanes wait until most of the wave is idle: 48 of
+// 64 for queued rays (the speed is flat from 40 to 60), and all 64 for camera
+// rays, whose wave is one pixel's samples with walks alike (the whole wave
+// takes its next 64 together: 3% faster than refilling at 56 once the QMC
+// jitter got cheap). Measured on the 1M-triangle icosphere at 1080p. Each
+// query still goes through the same steps, so the hits are the same.
 template <bool PRIMARY>
 struct refill_threshold {
-    static constexpr uint32_t lanes = PRIMARY ? 56 : 48;
+    static constexpr uint32_t lanes = PRIMARY ? 64 : 48;
 };
 
 // Scenes with instances walk with lane refill too (the instance tree's steps
