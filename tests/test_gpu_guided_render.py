@@ -75,6 +75,30 @@ def test_guided_render_is_deterministic():
     assert not np.array_equal(a, e)
 
 
+def test_guide_stats():
+    """guide_queries (PT_RENDER_COUNT) counts the diffuse hits the network was evaluated for: none in a plain
+    render; at one bounce in the closed config-4 room, nearly every camera ray (the room's surfaces are matte,
+    the lamps tiny); more with more bounces; the same on a second render. guide_ms (PT_RENDER_TIMING) is the
+    inference launches' share of the render."""
+    ptgp = pt.pack(pt.generate(4, 1, 0.02))
+    W, H, spp = 64, 48, 4
+    with pt.Scene(ptgp) as s:
+        g = pt.Guide(s, seed=3)
+        s.set_guide(g)
+        _, _, plain = s.render(W, H, spp, seed=1, count=True)
+        _, _, one = s.render(W, H, spp, seed=1, guide=True, count=True, max_bounces=1)
+        _, _, full = s.render(W, H, spp, seed=1, guide=True, count=True)
+        _, _, again = s.render(W, H, spp, seed=1, guide=True, count=True)
+        _, _, timed = s.render(W, H, spp, seed=1, guide=True, timing=True)
+        g.close()
+    print(f"guide queries: plain {plain.guide_queries}, one bounce {one.guide_queries} of {W * H * spp} camera rays, "
+          f"ten bounces {full.guide_queries}; inference {timed.guide_ms:.3f} of {timed.render_ms:.3f} ms", flush=True)
+    assert plain.guide_queries == 0 and plain.guide_ms == 0
+    assert 0.9 * W * H * spp <= one.guide_queries <= W * H * spp
+    assert full.guide_queries > one.guide_queries and full.guide_queries == again.guide_queries
+    assert 0 < timed.guide_ms < timed.render_ms
+
+
 def test_guided_render_needs_a_guide():
     with pt.Scene(rc.load_scene("room")[1]) as s:
         with pytest.raises(pt.PTError):
