@@ -147,13 +147,13 @@ __device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
 }
 __device__ __forceinline__ float vmf_pdf(const Lobe& l, f3 w) { return vmf_pdf(l.mu, l.kappa, w); }
 
-// The mixture as the path loop samples and evaluates it (and pt_guide_pdf /
-// pt_guide_sample): the hardware's fast exponential, logarithm, sine and
-// cosine (arguments here are bounded: angles in (0, 2 pi), exponents at most
-// 2 e^10), where the trainer's gradient keeps the correctly rounded library
-// functions. The sampler and the pdf share these functions and the lobe axis
-// is renormalised, so a guided sample is weighted by the density it was
-// drawn from, and the estimator stays unbiased.
+// The mixture from the network's outputs as pt_guide_pdf / pt_guide_sample
+// evaluate it (the path loop decodes the same lobes once per hit, below):
+// the hardware's fast exponential, logarithm, sine and cosine (arguments here
+// are bounded: angles in (0, 2 pi), exponents at most 2 e^10), where the
+// trainer's gradient keeps the correctly rounded library functions. The
+// sampler and the pdf share these functions and the lobe axis is
+// renormalised, so a sample is weighted by the density it was drawn from.
 __device__ __forceinline__ float fast_sigmoid(float v) { return __fdividef(1.0f, 1.0f + __expf(-v)); }
 
 __device__ __forceinline__ float fast_vmf_pdf(f3 mu, float kappa, f3 w) {
@@ -201,10 +201,24 @@ __device__ __forceinline__ float mixture_pdf(const T* o, f3 w) {
     return __fdividef(p, sum);
 }
 
+// A direction from lobe (nn, kappa): the inversion c = 1 + log(u1 + (1 - u1)
+// e^{-2 kappa}) / kappa for the cosine to the axis, a uniform azimuth u2.
+__device__ __forceinline__ f3 vmf_sample(f3 nn, float kappa, float u1, float u2) {
+    const float cw = fminf(1.0f, fmaxf(-1.0f, 1 + __fdividef(__logf(u1 + (1 - u1) * __expf(-2 * kappa)), kappa)));
+    const float s = sqrtf(fmaxf(0.0f, 1 - cw * cw));
+    const float phi = 2 * PI_F * u2;
+    float sphi, cphi;
+    __sincosf(phi, &sphi, &cphi);
+    const float sign = copysignf(1.0f, nn.z);
+    const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
+    const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
+    const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
+    return t1 * (s * cphi) + t2 * (s * sphi) + nn * cw;
+}
+
 // A direction from the mixture: u0 picks the lobe by weight (the first whose
 // running weight passes u0, the last if rounding leaves u0 above the sum),
-// (u1, u2) sample it: the inversion c = 1 + log(u1 + (1 - u1) e^{-2 kappa}) /
-// kappa for the cosine to the lobe axis, a uniform azimuth.
+// (u1, u2) sample it.
 template <class T>
 __device__ __forceinline__ f3 mixture_sample(const T* o, float u0, float u1, float u2) {
     float mx, sum;
@@ -224,16 +238,75 @@ __device__ __forceinline__ f3 mixture_sample(const T* o, float u0, float u1, flo
     f3 nn;
     float kappa;
     lobe_at(o, sel, nn, kappa);
-    const float cw = fminf(1.0f, fmaxf(-1.0f, 1 + __fdividef(__logf(u1 + (1 - u1) * __expf(-2 * kappa)), kappa)));
-    const float s = sqrtf(fmaxf(0.0f, 1 - cw * cw));
-    const float phi = 2 * PI_F * u2;
-    float sphi, cphi;
-    __sincosf(phi, &sphi, &cphi);
-    const float sign = copysignf(1.0f, nn.z);
-    const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
-    const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
-    const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
-    return t1 * (s * cphi) + t2 * (s * sphi) + nn * cw;
+    return vmf_sample(nn, kappa, u1, u2);
+}
+
+// ---------------------------------------------------------------------------
+// The mixture as the path loop carries it from the inference stage to the
+// shade stage: per lobe four FP16 values (x, y, s, w) in place of the lobe's
+// four outputs -- the axis' first two components, kappa with the sign of
+// the axis' third component, and the lobe's softmax weight. The inference
+// kernel decodes each lobe once (its lanes hold whole lobes); the shade
+// stage's sampler and pdf then read the same rounded values, rebuild the
+// axis from them the same way, and weight by the density they sample, so the
+// estimator stays unbiased.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void decode_lobe(float o0, float o1, float o2, float& x, float& y, float& s) {
+    const float th = PI_F * fast_sigmoid(o0), ph = 2 * PI_F * fast_sigmoid(o1);
+    float st, ct, sp, cp;
+    __sincosf(th, &st, &ct);
+    __sincosf(ph, &sp, &cp);
+    x = st * cp;
+    y = st * sp;
+    s = copysignf(__expf(fminf(fmaxf(o2, LOG_KAPPA_MIN), LOG_KAPPA_MAX)), ct);
+}
+
+// Lobe k of a decoded mixture: its unit axis, kappa and weight.
+__device__ __forceinline__ void decoded_lobe(const _Float16* q, int k, f3& mu, float& kappa, float& w) {
+    const float x = static_cast<float>(q[4 * k]), y = static_cast<float>(q[4 * k + 1]);
+    const float s = static_cast<float>(q[4 * k + 2]);
+    w = static_cast<float>(q[4 * k + 3]);
+    const f3 m = mk(x, y, copysignf(sqrtf(fmaxf(0.0f, 1 - x * x - y * y)), s));
+    mu = m * rsqrtf(dot(m, m));
+    kappa = fabsf(s);
+}
+
+__device__ __forceinline__ float decoded_weight_sum(const _Float16* q) {
+    float sum = 0;
+#pragma unroll
+    for (int k = 0; k < LOBES; ++k) sum += static_cast<float>(q[4 * k + 3]);
+    return sum;
+}
+
+__device__ __forceinline__ float decoded_pdf(const _Float16* q, f3 w) {
+    float p = 0;
+#pragma unroll 1
+    for (int k = 0; k < LOBES; ++k) {
+        f3 mu;
+        float kappa, wk;
+        decoded_lobe(q, k, mu, kappa, wk);
+        p += wk * fast_vmf_pdf(mu, kappa, w);
+    }
+    return __fdividef(p, decoded_weight_sum(q));
+}
+
+__device__ __forceinline__ f3 decoded_sample(const _Float16* q, float u0, float u1, float u2) {
+    const float target = u0 * decoded_weight_sum(q);
+    float acc = 0;
+    int sel = LOBES - 1;
+    bool chosen = false;
+#pragma unroll
+    for (int k = 0; k < LOBES; ++k) {
+        acc += static_cast<float>(q[4 * k + 3]);
+        if (!chosen && target < acc) {
+            sel = k;
+            chosen = true;
+        }
+    }
+    f3 nn;
+    float kappa, wk;
+    decoded_lobe(q, sel, nn, kappa, wk);
+    return vmf_sample(nn, kappa, u1, u2);
 }
 
 // Gradient of the sample's loss ws * -log p(w) with respect to the 32

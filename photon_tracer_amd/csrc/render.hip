@@ -480,12 +480,12 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
             // any guide. A direction below the surface carries nothing on.
             if (rand01(ps.key, ps.cnt) >= 0.5f) {
                 const float u0 = rand01(ps.key, ps.cnt), u1 = rand01(ps.key, ps.cnt), u2 = rand01(ps.key, ps.cnt);
-                dir = guide::mixture_sample(go, u0, u1, u2);
+                dir = guide::decoded_sample(go, u0, u1, u2);
             }
             const float cosv = dot(dir, nrm);
             if (cosv > 0) {
                 const float pc = cosv * (1 / PI_F);
-                const float f = pc / (0.5f * (pc + guide::mixture_pdf(go, dir)));
+                const float f = pc / (0.5f * (pc + guide::decoded_pdf(go, dir)));
                 wgt = att * f;
                 ps.mis *= f;
                 if (fp.guide_mode == 2 && ps.nrec < GUIDE_RECORDS) {
@@ -596,9 +596,10 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
     return r;
 }
 
-// The guide's outputs for the shade stage of bounce b: for every queued hit
+// The guide's mixture for the shade stage of bounce b: for every queued hit
 // on a diffuse (Lambertian) surface, the network at its point and normal,
-// 32 FP16 values at gout[slot]. The block first copies the network's weights
+// decoded to 32 FP16 values at gout[slot] (per lobe: axis x, y, kappa with
+// the axis' z sign, weight). The block first copies the network's weights
 // into LDS; then each wave takes 64 slots at a time and keeps them in
 // registers from the encoding to the outputs (no barrier after the copy);
 // waves with no diffuse hit skip the network. Weights read from the caches
@@ -645,16 +646,32 @@ __global__ __launch_bounds__(INFER_BLOCK) __attribute__((amdgpu_waves_per_eu(4))
         guide::encode_regs(fp.gv, pnt, nrm, X);
         guide::f16v O[2];
         guide::forward_regs(guide::LdsWeights{net}, X, O);
+        // The lane holds whole lobes (2 g4 + kh, g4 < 4) of queries c and
+        // 32 + c: each is decoded once here for the shade stage, the softmax
+        // taking the other half's four lobes from the partner lane.
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
+            float mx = O[nb][3];
+#pragma unroll
+            for (int g4 = 1; g4 < 4; ++g4) mx = fmaxf(mx, O[nb][4 * g4 + 3]);
+            mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+            float e[4], sum = 0;
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                e[g4] = __expf(O[nb][4 * g4 + 3] - mx);
+                sum += e[g4];
+            }
+            sum += __shfl_xor(sum, 32, 64);
+            const float inv = __fdividef(1.0f, sum);
             const int i = 32 * nb + c;
             if ((rows >> i) & 1ull) {
                 _Float16* out = fp.gout + static_cast<size_t>(base + i) * guide::OUT + 4 * kh;
 #pragma unroll
                 for (int g4 = 0; g4 < 4; ++g4) {
-                    guide::h4 hv;
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) hv[t] = static_cast<_Float16>(O[nb][4 * g4 + t]);
+                    float x, y, sk;
+                    guide::decode_lobe(O[nb][4 * g4], O[nb][4 * g4 + 1], O[nb][4 * g4 + 2], x, y, sk);
+                    const guide::h4 hv{static_cast<_Float16>(x), static_cast<_Float16>(y), static_cast<_Float16>(sk),
+                                       static_cast<_Float16>(e[g4] * inv)};
                     *reinterpret_cast<guide::h4*>(out + 8 * g4) = hv;
                 }
             }
@@ -675,7 +692,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1, nxt = cur ^ 1;
     __shared__ BlockAppend<3> sh;
-    // GUIDED: each thread's 32 network outputs, copied from HBM with four
+    // GUIDED: each thread's decoded mixture (32 FP16 values), copied from HBM with four
     // 16-byte loads into an LDS row of 17 words (an odd stride: the lobe loops'
     // reads, one output at a time, fall in distinct banks), instead of
     // gathering them two bytes at a time from HBM inside the lobe loops.
