@@ -396,40 +396,6 @@ __device__ __forceinline__ void layer_forward(const GuideView& g, int l, const _
     }
 }
 
-// The output layer of a wave's 64 rows X straight to HBM as FP16: row i (a
-// bit of `rows`) to out + i * OUT, four 8-byte runs per lane.
-__device__ __forceinline__ void output_store(const GuideView& g, const _Float16* X, unsigned long long rows,
-                                             _Float16* out) {
-    const int c = lane_c(), kh = lane_kh();
-    const _Float16* W = g.w16 + w_off(4);
-    const float* B = g.p32 + b_off(4);
-    h8 a[4];
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) a[kb] = *reinterpret_cast<const h8*>(W + c * IN + kb * 16 + kh * 8);
-#pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-        f16v acc;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = 0;
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            const h8 bx = *reinterpret_cast<const h8*>(X + (nb * 32 + c) * XS + kb * 16 + kh * 8);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[kb], bx, acc, 0, 0, 0);
-        }
-        const int i = nb * 32 + c;
-        if ((rows >> i) & 1ull) {
-#pragma unroll
-            for (int g4 = 0; g4 < 4; ++g4) {
-                const int m0 = 8 * g4 + 4 * kh;
-                h4 hv;
-#pragma unroll
-                for (int t = 0; t < 4; ++t) hv[t] = static_cast<_Float16>(acc[4 * g4 + t] + B[m0 + t]);
-                *reinterpret_cast<h4*>(out + i * OUT + m0) = hv;
-            }
-        }
-    }
-}
-
 // The whole network on a wave's 64 encoded rows X[0]; hidden activations in
 // X[1..4] (and their transposes in XT[0..4] when XT is not null); outputs in O.
 __device__ __forceinline__ void forward(const GuideView& g, _Float16* const* X, _Float16* const* XT, float* O) {
