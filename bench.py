@@ -321,6 +321,35 @@ def configs_leg(pt, cpu_seconds, frames=3):
     return out
 
 
+def denoise_leg(pt):
+    """The denoiser (the reference's filter as the CLI's --denoise step runs
+    it) on a 1080p render of the reference's room at 16 spp with aux: the
+    device time against the reference's own filter on the host's threads,
+    and the largest relative difference between the two images."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    import render_check as rc
+
+    W, H, spp = 1920, 1080, 16
+    with pt.Scene(rc.load_scene("room")[1]) as s:
+        rgb, aux, _ = s.render(W, H, spp, seed=3, aux=True)
+    pt.denoise(rgb, aux, spp)   # warmup
+    runs = [pt.denoise(rgb, aux, spp) for _ in range(3)]
+    ours, ms = runs[-1][0], min(r[1] for r in runs)
+    out = {"workload": f"the reference's room, {W}x{H}, {spp} spp with aux", "device_ms": round(ms, 3)}
+    if ob.ref_available():
+        # The CPU baseline: the reference's filter itself, on the same frame.
+        threads = max(1, min(16, os.cpu_count() or 1))
+        ref, secs = ob.ref_denoise(rgb, aux, spp, threads)
+        out["cpu_baseline"] = {"value": round(secs * 1e3, 1), "unit": "ms", "cores": threads, "kind": "reference",
+                               "sample": f"the full {W}x{H} frame, rows split over {threads} threads as "
+                                         "denoise.hpp's for_rows does"}
+        out["max_rel_diff_vs_reference"] = float(np.max(np.abs(ours - ref) / np.maximum(np.abs(ref), 1e-3)))
+    log(f"denoise: {json.dumps(out)}")
+    return out
+
+
 def guided_kernels(pt):
     """Plain and guided 16-spp 1080p renders of the room, twice each, after
     four training batches: the kernels a profiler should see per render."""
@@ -362,7 +391,8 @@ def main():
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--guided-plain-spp", type=int, default=1024,
                     help="the equal-time budget: the plain render at this many spp")
-    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels", "configs"], default=None,
+    ap.add_argument("--no-denoise", action="store_true", help="skip the denoiser leg")
+    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels", "configs", "denoise"], default=None,
                     help="run one leg alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also copy MIB MiB device to device (a kernel of known traffic, to scale HBM counters)")
@@ -379,7 +409,8 @@ def main():
                                           not args.batch_eager),
                "guided": lambda: guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp),
                "guided-kernels": lambda: guided_kernels(pt),
-               "configs": lambda: configs_leg(pt, args.cpu_seconds)}[args.only]
+               "configs": lambda: configs_leg(pt, args.cpu_seconds),
+               "denoise": lambda: denoise_leg(pt)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
 
@@ -461,6 +492,7 @@ def main():
     batch = None if args.no_batch else batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
                                                   not args.batch_eager)
     guided = None if args.no_guided else guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp)
+    denoised = None if args.no_denoise else denoise_leg(pt)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -475,6 +507,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "guided": guided,
+        "denoise": denoised,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)

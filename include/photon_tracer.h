@@ -193,6 +193,24 @@ typedef struct {
 pt_status pt_render(pt_scene* scene, const pt_render_params* params, float* rgb, float* aux,
                     pt_render_stats* stats);
 
+/* ---------------------------------------------------------------------------
+ * Denoising: the CLI's --denoise step (main.cpp:172-186) and the reference's
+ * filter (denoise.hpp:55-139): an edge-avoiding a-trous wavelet filter over
+ * the lighting (color divided by the first-hit albedo), steered by each
+ * pixel's noise, five passes of a 5x5 kernel with taps 1 to 16 pixels apart.
+ * rgb: 3 floats a pixel as pt_render writes them; aux: pt_render's 7 aux
+ * floats a pixel (albedo, normal, mean squared luminance); spp: the samples
+ * behind them (each pixel's variance of the mean is the sample variance over
+ * spp). out: 3 floats a pixel, not aliasing rgb. With PT_DENOISE_DEVICE all
+ * three are device addresses; otherwise host memory, copied over and back.
+ * device_ms (optional): the kernels' time. Working buffers (72 bytes a pixel,
+ * and 52 more for the copies of host images) are allocated and freed by each
+ * call.
+ * ------------------------------------------------------------------------- */
+enum { PT_DENOISE_DEVICE = 1 };
+pt_status pt_denoise(int width, int height, int spp, const float* rgb, const float* aux, float* out, int flags,
+                     double* device_ms);
+
 /* Device memory for callers without the HIP headers (ray and hit buffers for
  * pt_trace_device, images for PT_RENDER_DEVICE_OUTPUT). No counterpart in the
  * reference, which has no device. */

@@ -21,6 +21,7 @@
 #include <thread>
 #include <vector>
 
+#include "denoise.hpp"
 #include "gpu_pack.hpp"
 #include "renderer.hpp"
 
@@ -459,5 +460,28 @@ int ref_sample_2d(uint64_t seed, int i, int j, uint32_t index, int pair, float* 
 
 // std::thread::hardware_concurrency(), as main.cpp:116-118 uses.
 int ref_hardware_threads() { return static_cast<int>(std::thread::hardware_concurrency()); }
+
+// The CLI's --denoise step (main.cpp:172-186): albedo and normal from the
+// render's aux floats, each pixel's variance of the mean from the mean
+// squared luminance, then the reference's filter (denoise.hpp) on `threads`
+// threads. rgb: 3 floats a pixel, aux: aux_floats a pixel, out: 3 a pixel.
+int ref_denoise(int width, int height, int spp, const float* rgb, const float* aux, float* out, int threads,
+                double* seconds) {
+    const auto start = std::chrono::steady_clock::now();
+    const size_t n = static_cast<size_t>(width) * height;
+    std::vector<float> albedo(3 * n), normal(3 * n), variance(n);
+    for (size_t p = 0; p < n; ++p) {
+        const float* a = &aux[aux_floats * p];
+        for (int c = 0; c < 3; ++c) {
+            albedo[3 * p + c] = a[c];
+            normal[3 * p + c] = a[3 + c];
+        }
+        const float mean = luminance(color(rgb[3 * p], rgb[3 * p + 1], rgb[3 * p + 2]));
+        variance[p] = (a[6] - mean * mean) / spp;
+    }
+    denoise({width, height, rgb, albedo.data(), normal.data(), variance.data()}, out, threads);
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return 0;
+}
 
 }  // extern "C"

@@ -148,6 +148,8 @@ def library() -> ctypes.CDLL:
     lib.pt_trace_device.argtypes = [vp, vp, sz, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_float),
                                     ctypes.POINTER(ctypes.c_uint64)]
     lib.pt_render.argtypes = [vp, ctypes.POINTER(_RenderParams), vp, vp, ctypes.POINTER(_RenderStats)]
+    lib.pt_denoise.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_double)]
     lib.pt_device_alloc.argtypes = [sz, ctypes.POINTER(vp)]
     lib.pt_device_free.argtypes = [vp]
     lib.pt_device_copy.argtypes = [vp, vp, sz, ctypes.c_int]
@@ -369,6 +371,20 @@ class Scene:
         stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
                             tuple(st.visits), st.batches, st.samples_per_batch, st.guide_ms, st.guide_queries)
         return rgb, ax, stats
+
+
+def denoise(rgb: np.ndarray, aux: np.ndarray, spp: int) -> tuple[np.ndarray, float]:
+    """The reference's denoiser (denoise.hpp, as main.cpp's --denoise step feeds it) on the device, over a
+    render's rgb (h, w, 3) and aux (h, w, 7) outputs from `spp` samples. Returns (image, device_ms)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    aux = np.ascontiguousarray(aux, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    assert rgb.shape == (h, w, 3) and aux.shape == (h, w, 7)
+    out = np.zeros_like(rgb)
+    ms = ctypes.c_double(0)
+    _check(library().pt_denoise(w, h, int(spp), rgb.ctypes.data, aux.ctypes.data, out.ctypes.data, 0,
+                                ctypes.byref(ms)))
+    return out, ms.value
 
 
 def _f32(a, cols: int) -> np.ndarray:

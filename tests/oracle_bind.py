@@ -200,6 +200,21 @@ class RefWorld:
         return rgb, ax, int(rays.value), float(secs.value)
 
 
+def ref_denoise(rgb, aux, spp, threads: int = THREADS):
+    """The reference's --denoise step (main.cpp:172-186, denoise.hpp) on (h, w, 3) rgb and (h, w, 7) aux.
+    Returns (image, seconds)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    aux = np.ascontiguousarray(aux, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    out = np.zeros_like(rgb)
+    secs = ctypes.c_double(0)
+    lib = ref()
+    lib.ref_denoise.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.ref_denoise(w, h, int(spp), rgb.ctypes.data, aux.ctypes.data, out.ctypes.data, threads, ctypes.byref(secs))
+    return out, secs.value
+
+
 def ref_sample_2d(seed, i, j, index, pair):
     a, b = ctypes.c_float(), ctypes.c_float()
     ref().ref_sample_2d(seed, i, j, index, pair, ctypes.byref(a), ctypes.byref(b))
