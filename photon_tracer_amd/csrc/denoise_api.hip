@@ -49,17 +49,21 @@ pt_status pt_denoise(int width, int height, int spp, const float* rgb, const flo
                                      {var[0].as<float>(), var[1].as<float>()},
                                      normal.as<float4>(),
                                      albedo.as<float4>()};
-        hipEvent_t e0, e1;
-        HIPC(hipEventCreate(&e0));
-        HIPC(hipEventCreate(&e1));
-        HIPC(hipEventRecord(e0, nullptr));
+        struct events {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~events() {
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } ev;
+        HIPC(hipEventCreate(&ev.e[0]));
+        HIPC(hipEventCreate(&ev.e[1]));
+        HIPC(hipEventRecord(ev.e[0], nullptr));
         HIPC(ptd::dn::launch_denoise(width, height, spp, in_rgb, in_aux, to, b, nullptr));
-        HIPC(hipEventRecord(e1, nullptr));
-        HIPC(hipEventSynchronize(e1));
+        HIPC(hipEventRecord(ev.e[1], nullptr));
+        HIPC(hipEventSynchronize(ev.e[1]));
         float ms = 0;
-        HIPC(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        HIPC(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
         if (device_ms) *device_ms = ms;
         if (!device) HIPC(hipMemcpy(out, dout.p, 12 * n, hipMemcpyDeviceToHost));
     });
