@@ -481,7 +481,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         fp.spill = s->spill.as<uint2>();
         // Below four rays per resident traversal lane, a queue no longer fills
         // the GPU stage by stage and the tail launch takes the paths.
-        fp.tail_threshold = (P.flags & PT_RENDER_NO_TAIL) ? 0u : static_cast<uint32_t>(trace_grid) * ptd::TRACE_BLOCK * 4u;
+        fp.tail_threshold = (P.flags & PT_RENDER_NO_TAIL) ? 0u : ptapi::tail_threshold(trace_grid, s->view.inst_root != NONE);
         if (guided) {
             fp.guide_mode = training ? 2 : 1;
             fp.gv = G->view();

@@ -339,7 +339,7 @@ pt_status pt_render_batch(pt_batch* b, const pt_batch_job* jobs, int n_jobs, con
             fp.totals = b->totals.as<unsigned long long>() + 4 * size_t(j);
             fp.spill = st.spill.as<uint2>();
             fp.tail_threshold =
-                (P.flags & PT_RENDER_NO_TAIL) ? 0u : static_cast<uint32_t>(trace_grid) * ptd::TRACE_BLOCK * 4u;
+                (P.flags & PT_RENDER_NO_TAIL) ? 0u : ptapi::tail_threshold(trace_grid, false);   // batched scenes are flat
             return fp;
         };
         // A stream's whole sequence: for each of its jobs, clear, the

@@ -101,6 +101,12 @@ __device__ __forceinline__ bool sphere_hit_d(float4 cr, f3 o, f3 d, float t_min,
     const double a = dx * dx + dy * dy + dz * dz;
     const double half_b = ox * dx + oy * dy + oz * dz;
     const double c = ox * ox + oy * oy + oz * oz - double(cr.w) * cr.w;
+    // Outside the sphere and moving away: both roots are at most 0, below
+    // any t_min > 0. In IEEE arithmetic disc <= fl(half_b^2) here and
+    // sqrt(fl(half_b^2)) = half_b, so the full test would reject both roots
+    // too; this skips its square root and divisions (the sky half of a
+    // ground sphere's queries).
+    if (half_b >= 0 && c > 0 && t_min > 0) return false;
     const double disc = half_b * half_b - a * c;
     if (disc < 0) return false;
     const double sq = sqrt(disc);
