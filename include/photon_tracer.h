@@ -194,6 +194,25 @@ pt_status pt_render(pt_scene* scene, const pt_render_params* params, float* rgb,
                     pt_render_stats* stats);
 
 /* ---------------------------------------------------------------------------
+ * Progressive rendering: the browser demo's driver (gpu.js:199-231) over
+ * pt_render's first_sample continuation (renderer.hpp:345-356). A render of
+ * params->spp samples is taken in steps that start at one sample per pixel
+ * and double while a step's device time stays under 20 ms (up to 64), halving
+ * above 60 ms; the running sum stays in HBM, so each step costs its own
+ * samples only. Each sample is the one a single pt_render of params->spp
+ * samples would draw (same seed, pixel and sample number).
+ * ------------------------------------------------------------------------- */
+typedef struct pt_progressive pt_progressive;
+/* params as for pt_render (spp: the total); the scene must outlive the handle. */
+pt_status pt_progressive_create(pt_scene* scene, const pt_render_params* params, pt_progressive** out);
+/* Renders the next step. done: samples per pixel rendered so far (== spp when
+ * the render is complete: further steps are refused); step_spp: this step's.
+ * rgb: NULL, or the running average, 3 floats a pixel (a device address with
+ * PT_RENDER_DEVICE_OUTPUT in params->flags). stats: this step's, optional. */
+pt_status pt_progressive_step(pt_progressive* p, float* rgb, int* done, int* step_spp, pt_render_stats* stats);
+pt_status pt_progressive_destroy(pt_progressive* p);
+
+/* ---------------------------------------------------------------------------
  * Denoising: the CLI's --denoise step (main.cpp:172-186) and the reference's
  * filter (denoise.hpp:55-139): an edge-avoiding a-trous wavelet filter over
  * the lighting (color divided by the first-hit albedo), steered by each

@@ -150,6 +150,10 @@ def library() -> ctypes.CDLL:
     lib.pt_render.argtypes = [vp, ctypes.POINTER(_RenderParams), vp, vp, ctypes.POINTER(_RenderStats)]
     lib.pt_denoise.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int,
                                ctypes.POINTER(ctypes.c_double)]
+    lib.pt_progressive_create.argtypes = [vp, ctypes.POINTER(_RenderParams), ctypes.POINTER(vp)]
+    lib.pt_progressive_step.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(_RenderStats)]
+    lib.pt_progressive_destroy.argtypes = [vp]
     lib.pt_device_alloc.argtypes = [sz, ctypes.POINTER(vp)]
     lib.pt_device_free.argtypes = [vp]
     lib.pt_device_copy.argtypes = [vp, vp, sz, ctypes.c_int]
@@ -371,6 +375,39 @@ class Scene:
         stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
                             tuple(st.visits), st.batches, st.samples_per_batch, st.guide_ms, st.guide_queries)
         return rgb, ax, stats
+
+
+class Progressive:
+    """A progressive render (pt_progressive_*): `spp` samples in adaptive steps (1, 2, 4 ... up to 64 while a
+    step stays under 20 ms), as the reference's browser demo renders. step() returns (done, step_spp, image of
+    the running average)."""
+
+    def __init__(self, scene: "Scene", width: int, height: int, spp: int, max_bounces: int = 10, seed: int = 0,
+                 first_sample: int = 0, nee: bool = True):
+        self.width, self.height, self.spp = width, height, spp
+        p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, 0, 0, 0 if nee else RENDER_NO_NEE, 0, 0)
+        h = ctypes.c_void_p()
+        _check(library().pt_progressive_create(scene._h, ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.done = 0
+
+    def step(self):
+        img = np.zeros((self.height, self.width, 3), np.float32)
+        done, n = ctypes.c_int(0), ctypes.c_int(0)
+        _check(library().pt_progressive_step(self._h, img.ctypes.data, ctypes.byref(done), ctypes.byref(n), None))
+        self.done = done.value
+        return done.value, n.value, img
+
+    def close(self):
+        if self._h:
+            library().pt_progressive_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def denoise(rgb: np.ndarray, aux: np.ndarray, spp: int) -> tuple[np.ndarray, float]:

@@ -97,6 +97,8 @@ constexpr int TRACE_BLOCK = 256;
 hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
                         hipEvent_t* ev);
 hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStream_t stream);
+hipError_t launch_progressive_add(const float* img, float* sum, uint32_t npix, int n, bool first, hipStream_t stream);
+hipError_t launch_progressive_resolve(const float* sum, float* rgb, uint32_t npix, int done, hipStream_t stream);
 // After a training batch: its records' weights, compacted in path order into
 // fp.gsamples (count and weight sum in fp.gtot).
 hipError_t launch_guide_targets(const FrameParams& fp, hipStream_t stream);

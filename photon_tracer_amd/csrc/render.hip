@@ -1230,6 +1230,33 @@ hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid
     return hipGetLastError();
 }
 
+// Progressive renders: a step's image (the average of its n samples, 3
+// floats a pixel) added into the running sum, and the running average out.
+__global__ __launch_bounds__(256) void k_prog_add(const float* img, float* sum, uint32_t n3, float n, int first) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n3) return;
+    const float v = img[k] * n;
+    sum[k] = first ? v : sum[k] + v;
+}
+
+__global__ __launch_bounds__(256) void k_prog_resolve(const float* sum, float* rgb, uint32_t n3, float done) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n3) return;
+    rgb[k] = sum[k] / done;
+}
+
+hipError_t launch_progressive_add(const float* img, float* sum, uint32_t npix, int n, bool first, hipStream_t stream) {
+    const uint32_t n3 = 3 * npix;
+    k_prog_add<<<(n3 + 255) / 256, 256, 0, stream>>>(img, sum, n3, static_cast<float>(n), first ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_progressive_resolve(const float* sum, float* rgb, uint32_t npix, int done, hipStream_t stream) {
+    const uint32_t n3 = 3 * npix;
+    k_prog_resolve<<<(n3 + 255) / 256, 256, 0, stream>>>(sum, rgb, n3, static_cast<float>(done));
+    return hipGetLastError();
+}
+
 hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStream_t stream) {
     k_resolve<<<(fp.npix + 255) / 256, 256, 0, stream>>>(fp, rgb, aux);
     return hipGetLastError();

@@ -145,3 +145,25 @@ def test_bad_render_parameters_are_refused():
                    {"width": 10, "height": 10, "spp": 4, "stack_entries": 12}):
             with pytest.raises(pt.PTError):
                 s.render(**kw)
+
+
+def test_progressive_driver_equals_one_render():
+    """pt_progressive_*: the samples of a progressive render are those of one render of the same total (same
+    seed, pixel and sample numbers), so the running average after the last step equals that render up to the
+    rounding of summing per-step averages; steps start at one sample, are powers of two up to 64, and add up
+    to the total; a step after the last is refused."""
+    ptgp = rc.load_scene("field")[1]
+    W, H, spp = 96, 64, 100
+    with pt.Scene(ptgp) as s:
+        want, _, _ = s.render(W, H, spp, seed=7)
+        steps, img = [], None
+        with pt.Progressive(s, W, H, spp, seed=7) as prog:
+            while prog.done < spp:
+                done, n, img = prog.step()
+                steps.append(n)
+            with pytest.raises(pt.PTError):
+                prog.step()
+    print(f"steps {steps}", flush=True)
+    assert steps[0] == 1 and sum(steps) == spp
+    assert all(n & (n - 1) == 0 and n <= 64 for n in steps[:-1])
+    assert np.allclose(img, want, rtol=2e-5, atol=1e-6)
