@@ -350,6 +350,37 @@ def denoise_leg(pt):
     return out
 
 
+def bvh_leg(pt):
+    """BVH builds on the GPU against the host builder's threads: the whole
+    scene pack of the 1M-triangle frame's scene (pt_pack against pt_pack_ex
+    with the device builder; the rest of the pack is the same host code), and
+    the builder alone over 1M uniform boxes. Both checked word for word."""
+    import numpy as np
+    ptsd = pt.generate(1, 1, 1.0)
+    t = time.perf_counter()
+    host = pt.pack(ptsd)
+    host_ms = (time.perf_counter() - t) * 1e3
+    dev_ms, dev = [], None
+    for _ in range(3):   # the first call also loads the kernels
+        t = time.perf_counter()
+        dev = pt.pack(ptsd, device_bvh=True)
+        dev_ms.append((time.perf_counter() - t) * 1e3)
+    rng = np.random.default_rng(1)
+    c = rng.uniform(0, 1, (1_000_000, 3))
+    boxes = np.concatenate([c - 0.002, c + 0.002], 1).astype(np.float32)
+    hn, ho, hb_ms = pt.bvh_build(boxes)
+    db = [pt.bvh_build(boxes, device=True) for _ in range(2)]
+    out = {"pack_workload": "displaced icosphere scene, 1,003,520 triangles (pt_pack vs pt_pack_ex)",
+           "pack_host_ms": round(host_ms, 1), "pack_device_ms": round(min(dev_ms[1:]), 1),
+           "pack_identical": bool(np.array_equal(host, dev)),
+           "build_workload": "1,000,000 uniform boxes (pt_bvh_build)", "build_host_ms": round(hb_ms, 1),
+           "build_device_ms": round(min(d[2] for d in db), 1),
+           "build_identical": bool(np.array_equal(db[-1][0], hn) and np.array_equal(db[-1][1], ho)),
+           "host_threads": os.cpu_count()}
+    log(f"bvh: {json.dumps(out)}")
+    return out
+
+
 def guided_kernels(pt):
     """Plain and guided 16-spp 1080p renders of the room, twice each, after
     four training batches: the kernels a profiler should see per render."""
@@ -392,7 +423,8 @@ def main():
     ap.add_argument("--guided-plain-spp", type=int, default=1024,
                     help="the equal-time budget: the plain render at this many spp")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoiser leg")
-    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels", "configs", "denoise"], default=None,
+    ap.add_argument("--no-bvh", action="store_true", help="skip the BVH build leg")
+    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels", "configs", "denoise", "bvh"], default=None,
                     help="run one leg alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also copy MIB MiB device to device (a kernel of known traffic, to scale HBM counters)")
@@ -410,7 +442,8 @@ def main():
                "guided": lambda: guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp),
                "guided-kernels": lambda: guided_kernels(pt),
                "configs": lambda: configs_leg(pt, args.cpu_seconds),
-               "denoise": lambda: denoise_leg(pt)}[args.only]
+               "denoise": lambda: denoise_leg(pt),
+               "bvh": lambda: bvh_leg(pt)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
 
@@ -493,6 +526,7 @@ def main():
                                                   not args.batch_eager)
     guided = None if args.no_guided else guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp)
     denoised = None if args.no_denoise else denoise_leg(pt)
+    bvh = None if args.no_bvh else bvh_leg(pt)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -508,6 +542,7 @@ def main():
         "batch": batch,
         "guided": guided,
         "denoise": denoised,
+        "bvh_build": bvh,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)

@@ -86,6 +86,26 @@ pt_status pt_generate_job(uint64_t seed, double scale, pt_job_desc* desc, uint32
  * as PTGP v1 (pack_for_gpu + serialize, gpu_pack.hpp:84-203). */
 pt_status pt_pack(const uint32_t* ptsd, size_t n_words, uint32_t** ptgp, size_t* ptgp_words);
 
+/* pt_pack with options. PT_PACK_DEVICE_BVH builds every tree (mesh BLASes,
+ * the flat tree, the instance TLAS) with the GPU builder of pt_bvh_build; the
+ * PTGP words are the same as pt_pack's. */
+enum { PT_PACK_DEVICE_BVH = 1 };
+pt_status pt_pack_ex(const uint32_t* ptsd, size_t n_words, int flags, uint32_t** ptgp, size_t* ptgp_words);
+
+/* bvh::build over caller boxes (bvh.hpp:39-52): binned SAH with 16 bins,
+ * unit costs, max leaf 8, max depth 64, then the widest-first 4-wide collapse
+ * (bvh.hpp:126-247). boxes: n boxes of 6 floats (min xyz, max xyz: the aabb
+ * layout, aabb.hpp:9). Returns the 4-wide nodes (32 words each: lo[3][4],
+ * hi[3][4], first[4], count[4]; bvh.hpp:27-32) and the primitive order, both
+ * allocated by the library (free with pt_free); n = 0 gives no nodes.
+ * flags: PT_BVH_DEVICE builds on the GPU (level-parallel binning and
+ * partition, one wave per small subtree, parallel collapse) with the same
+ * result word for word; 0 builds on host threads. ms: wall-clock of the call,
+ * transfers included (may be NULL). At most 2^30 boxes. */
+enum { PT_BVH_DEVICE = 1 };
+pt_status pt_bvh_build(const float* boxes, size_t n, int flags, uint32_t** nodes, size_t* n_nodes, uint32_t** order,
+                       double* ms);
+
 /* Depth (levels of 4-wide nodes) of the deepest tree in a PTGP buffer. */
 pt_status pt_ptgp_depth(const uint32_t* ptgp, size_t n_words, int* depth);
 

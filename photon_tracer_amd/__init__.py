@@ -141,6 +141,9 @@ def library() -> ctypes.CDLL:
     lib.pt_generate_job.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(_JobDesc),
                                     ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
     lib.pt_pack.argtypes = [vp, sz, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
+    lib.pt_pack_ex.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
+    lib.pt_bvh_build.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz), ctypes.POINTER(_u32p),
+                                 ctypes.POINTER(ctypes.c_double)]
     lib.pt_ptgp_depth.argtypes = [vp, sz, ctypes.POINTER(ctypes.c_int)]
     lib.pt_scene_create.argtypes = [vp, sz, ctypes.POINTER(vp)]
     lib.pt_scene_destroy.argtypes = [vp]
@@ -178,7 +181,8 @@ def library() -> ctypes.CDLL:
     for name in ("pt_batch_merge", "pt_batch_create", "pt_batch_destroy", "pt_render_batch", "pt_batch_autotune"):
         getattr(lib, name).restype = ctypes.c_int
     for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate",
-                 "pt_generate_job", "pt_pack", "pt_ptgp_depth", "pt_scene_create", "pt_scene_destroy", "pt_trace",
+                 "pt_generate_job", "pt_pack", "pt_pack_ex", "pt_bvh_build", "pt_ptgp_depth", "pt_scene_create",
+                 "pt_scene_destroy", "pt_trace",
                  "pt_trace_device", "pt_render", "pt_guide_create", "pt_guide_destroy", "pt_scene_set_guide",
                  "pt_guide_get_weights", "pt_guide_set_weights", "pt_guide_eval", "pt_guide_pdf", "pt_guide_sample",
                  "pt_guide_gradient", "pt_guide_step", "pt_guide_stats"):
@@ -244,12 +248,28 @@ def _words(a) -> np.ndarray:
     return a
 
 
-def pack(ptsd) -> np.ndarray:
-    """Builds the BVHs of a PTSD scene and packs it as PTGP v1 words."""
+def pack(ptsd, device_bvh: bool = False) -> np.ndarray:
+    """Builds the BVHs of a PTSD scene and packs it as PTGP v1 words; with
+    device_bvh, the trees are built on the GPU (same words)."""
     w = _words(ptsd)
     p, n = _u32p(), ctypes.c_size_t(0)
-    _check(library().pt_pack(w.ctypes.data, w.size, ctypes.byref(p), ctypes.byref(n)))
+    if device_bvh:
+        _check(library().pt_pack_ex(w.ctypes.data, w.size, 1, ctypes.byref(p), ctypes.byref(n)))
+    else:
+        _check(library().pt_pack(w.ctypes.data, w.size, ctypes.byref(p), ctypes.byref(n)))
     return _take_words(p, n)
+
+
+def bvh_build(boxes, device: bool = False):
+    """bvh::build over boxes of shape (n, 6) float32 (min xyz, max xyz).
+    Returns (nodes as an (k, 32) uint32 array of 4-wide nodes, order, ms)."""
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    nodes, order = _u32p(), _u32p()
+    k, ms = ctypes.c_size_t(0), ctypes.c_double(0)
+    _check(library().pt_bvh_build(b.ctypes.data, b.shape[0], 1 if device else 0, ctypes.byref(nodes), ctypes.byref(k),
+                                  ctypes.byref(order), ctypes.byref(ms)))
+    words = _take_words(nodes, ctypes.c_size_t(32 * k.value)).reshape(-1, 32)
+    return words, _take_words(order, ctypes.c_size_t(b.shape[0])), ms.value
 
 
 def ptgp_depth(ptgp) -> int:

@@ -1,0 +1,948 @@
+// Device BVH builder. Same tree as bvh::build (bvh.hpp:39-52,126-247) and
+// the host builder, node for node and word for word:
+//
+//  * Segments longer than kSmall are split level by level across the whole
+//    GPU: each level bounds, bins and partitions every open segment, chunk by
+//    chunk (kChunk primitives per workgroup), with float min/max merged as
+//    order-preserving integer keys (min and max do not depend on the order
+//    they are taken in, so the boxes equal the reference's sequential grows).
+//  * Segments of at most kSmall primitives are finished by one wave each,
+//    depth first, with the segment's boxes and centroids held in LDS.
+//  * The 4-wide collapse (bvh.hpp:205-242) then runs over all binary nodes
+//    at once: each node's widest-first child list, the collapsed subtree
+//    sizes bottom-up and the reference's preorder numbering top-down, one
+//    launch per binary depth.
+//
+// The split arithmetic is the reference's, operation for operation, with no
+// FMA contraction (the build flags turn it off): bin_of, half_area and the
+// SAH cost round as they do on the host. std::partition's element order
+// (libstdc++'s two-ended swap for bidirectional iterators, which the
+// reference and the host builder run) is reproduced exactly: the k-th
+// element from the left that belongs right is swapped with the k-th element
+// from the right that belongs left, for every k up to the number of such
+// pairs.
+#include "bvh_gpu.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace ptd {
+namespace bvhg {
+namespace {
+
+constexpr int kBins = 16;           // bvh.hpp:116-120
+constexpr uint32_t kMaxLeaf = 8;
+constexpr uint32_t kMaxDepth = 64;
+constexpr float kTraversalCost = 1, kIntersectionCost = 1;
+
+constexpr uint32_t kSmall = 1024;   // longest segment one wave finishes in LDS
+constexpr uint32_t kChunk = 4096;   // primitives per workgroup in the level passes
+constexpr int kBlock = 256;
+constexpr int kBinWords = 3 * kBins * 7;   // per axis and bin: min xyz, max xyz (keys), count
+
+constexpr uint32_t KEY_POS_INF = 0xff800000u;   // fkey(+inf): the empty box's min
+constexpr uint32_t KEY_NEG_INF = 0x007fffffu;   // fkey(-inf): the empty box's max
+
+struct BNode {   // binary node (bvh.hpp:109-113)
+    float lo[3], hi[3];
+    uint32_t first, count;
+};
+
+struct alignas(16) WNode {   // 4-wide node (bvh.hpp:27-32)
+    float lo[3][4];
+    float hi[3][4];
+    uint32_t first[4];
+    uint32_t count[4];
+};
+static_assert(sizeof(WNode) == 128, "node layout");
+
+struct Seg {   // one open segment of a level pass
+    uint32_t first, count, depth, node;
+    uint32_t chunk_begin, chunk_count;
+    uint32_t kind;   // k_decide: 0 leaf, 1 partition on (axis, bin), 2 halve
+    int32_t axis, bin;
+    uint32_t mid, left;
+    float lo, extent;   // centroid bounds along the split axis
+    uint32_t pad[3];
+};
+
+struct SegAcc {
+    uint32_t key[12];   // box min, box max, centroid min, centroid max
+    uint32_t bins[kBinWords];
+};
+
+// Order-preserving map of floats to unsigned integers (no NaN arrives here).
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fval(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+struct FBox {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ void box_empty(FBox& b) {
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = __builtin_inff();
+        b.hi[a] = -__builtin_inff();
+    }
+}
+
+// aabb::grow (aabb.hpp:15-20): std::min(a, b) keeps a unless b < a.
+__device__ __forceinline__ void box_grow(FBox& b, const float* lo, const float* hi) {
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = lo[a] < b.lo[a] ? lo[a] : b.lo[a];
+        b.hi[a] = b.hi[a] < hi[a] ? hi[a] : b.hi[a];
+    }
+}
+
+// aabb::half_area (aabb.hpp:26-30).
+__device__ __forceinline__ float half_area(const FBox& b) {
+    if (b.lo[0] > b.hi[0]) return 0;
+    const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// bvh::bin_of (bvh.hpp:244-247).
+__device__ __forceinline__ int bin_of(float c, float lo, float extent) {
+    const int b = static_cast<int>(kBins * (c - lo) / extent);
+    return b < 0 ? 0 : b >= kBins ? kBins - 1 : b;
+}
+
+__device__ __forceinline__ void load_box(const float* __restrict__ boxes, uint32_t p, float* lo, float* hi) {
+    const float2* b = reinterpret_cast<const float2*>(boxes + 6 * size_t(p));
+    const float2 a = b[0], c = b[1], d = b[2];
+    lo[0] = a.x;
+    lo[1] = a.y;
+    lo[2] = c.x;
+    hi[0] = c.y;
+    hi[1] = d.x;
+    hi[2] = d.y;
+}
+
+// One axis of the SAH sweep (bvh.hpp:149-170) over its 16 bins (7 words
+// each), taking a boundary only when it is strictly cheaper than best_cost.
+__device__ __forceinline__ void sweep_axis(const uint32_t* bins, float parent_area, int axis, float& best_cost,
+                                           int& best_axis, int& best_bin) {
+    float right_area[kBins];
+    uint32_t right_count[kBins];
+    FBox acc;
+    box_empty(acc);
+    uint32_t n = 0;
+#pragma unroll
+    for (int b = kBins - 1; b > 0; --b) {
+        const uint32_t* w = bins + 7 * b;
+        const float lo[3] = {fval(w[0]), fval(w[1]), fval(w[2])};
+        const float hi[3] = {fval(w[3]), fval(w[4]), fval(w[5])};
+        box_grow(acc, lo, hi);
+        n += w[6];
+        right_area[b] = half_area(acc);
+        right_count[b] = n;
+    }
+    box_empty(acc);
+    n = 0;
+#pragma unroll
+    for (int b = 0; b < kBins - 1; ++b) {
+        const uint32_t* w = bins + 7 * b;
+        const float lo[3] = {fval(w[0]), fval(w[1]), fval(w[2])};
+        const float hi[3] = {fval(w[3]), fval(w[4]), fval(w[5])};
+        box_grow(acc, lo, hi);
+        n += w[6];
+        if (n == 0 || right_count[b + 1] == 0) continue;
+        const float cost = kTraversalCost + kIntersectionCost *
+                                                (half_area(acc) * n + right_area[b + 1] * right_count[b + 1]) /
+                                                parent_area;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best_axis = axis;
+            best_bin = b;
+        }
+    }
+}
+
+__device__ __forceinline__ float wave_min(float x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const float y = __shfl_xor(x, m, 64);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const float y = __shfl_xor(x, m, 64);
+        x = x < y ? y : x;
+    }
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
+    return x;
+}
+
+// Box and centroid bounds of 12 running values: [0,3) box min, [3,6) box
+// max, [6,9) centroid min, [9,12) centroid max.
+__device__ __forceinline__ void bounds_init(float* v) {
+    for (int a = 0; a < 3; ++a) {
+        v[a] = __builtin_inff();
+        v[3 + a] = -__builtin_inff();
+        v[6 + a] = __builtin_inff();
+        v[9 + a] = -__builtin_inff();
+    }
+}
+__device__ __forceinline__ void bounds_add(float* v, const float* lo, const float* hi, const float* c) {
+    for (int a = 0; a < 3; ++a) {
+        v[a] = lo[a] < v[a] ? lo[a] : v[a];
+        v[3 + a] = v[3 + a] < hi[a] ? hi[a] : v[3 + a];
+        v[6 + a] = c[a] < v[6 + a] ? c[a] : v[6 + a];
+        v[9 + a] = v[9 + a] < c[a] ? c[a] : v[9 + a];
+    }
+}
+__device__ __forceinline__ void bounds_wave(float* v) {
+    for (int a = 0; a < 3; ++a) {
+        v[a] = wave_min(v[a]);
+        v[3 + a] = wave_max(v[3 + a]);
+        v[6 + a] = wave_min(v[6 + a]);
+        v[9 + a] = wave_max(v[9 + a]);
+    }
+}
+
+__device__ __forceinline__ uint32_t bin_init_word(int i) {
+    const int k = i % 7;
+    return k < 3 ? KEY_POS_INF : k < 6 ? KEY_NEG_INF : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Set-up: centroids (aabb::center, aabb.hpp:22), identity order, counters.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_setup(const float* __restrict__ boxes, uint32_t n, float4* centers,
+                                                   uint32_t* order, uint32_t* counters) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) {
+        counters[0] = 1;   // node 0 is the root
+        counters[1] = 0;   // deepest binary depth
+    }
+    if (i >= n) return;
+    float lo[3], hi[3];
+    load_box(boxes, i, lo, hi);
+    centers[i] = make_float4((lo[0] + hi[0]) * 0.5f, (lo[1] + hi[1]) * 0.5f, (lo[2] + hi[2]) * 0.5f, 0.f);
+    order[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Level passes over segments longer than kSmall.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_seg_init(SegAcc* acc) {
+    SegAcc& a = acc[blockIdx.x];
+    for (int i = threadIdx.x; i < 12; i += kBlock) a.key[i] = (i % 6) < 3 ? KEY_POS_INF : KEY_NEG_INF;
+    for (int i = threadIdx.x; i < kBinWords; i += kBlock) a.bins[i] = bin_init_word(i);
+}
+
+__device__ __forceinline__ uint32_t chunk_end(const uint2& ch, const Seg& s) {
+    return min(ch.y + kChunk, s.first + s.count);
+}
+
+// bvh.hpp:128-132: the segment's box and centroid bounds.
+__global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ boxes, const float4* __restrict__ centers,
+                                                    const uint32_t* __restrict__ order, const uint2* __restrict__ chunks,
+                                                    const Seg* __restrict__ segs, SegAcc* acc) {
+    __shared__ float part[kBlock / 64][12];
+    const uint2 ch = chunks[blockIdx.x];
+    const Seg s = segs[ch.x];
+    const uint32_t end = chunk_end(ch, s);
+    float v[12];
+    bounds_init(v);
+    for (uint32_t i = ch.y + threadIdx.x; i < end; i += kBlock) {
+        const uint32_t p = order[i];
+        float lo[3], hi[3];
+        load_box(boxes, p, lo, hi);
+        const float4 c4 = centers[p];
+        const float c[3] = {c4.x, c4.y, c4.z};
+        bounds_add(v, lo, hi, c);
+    }
+    bounds_wave(v);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int k = 0; k < 12; ++k) part[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        const bool is_min = (k % 6) < 3;
+        float x = part[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            const float y = part[w][k];
+            x = is_min ? (y < x ? y : x) : (x < y ? y : x);
+        }
+        if (is_min)
+            atomicMin(&acc[ch.x].key[k], fkey(x));
+        else
+            atomicMax(&acc[ch.x].key[k], fkey(x));
+    }
+}
+
+// bvh.hpp:141-148: per axis with a positive centroid extent, the 16 bins'
+// boxes and counts.
+__global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ boxes, const float4* __restrict__ centers,
+                                                 const uint32_t* __restrict__ order, const uint2* __restrict__ chunks,
+                                                 const Seg* __restrict__ segs, SegAcc* acc) {
+    __shared__ uint32_t bins[kBinWords];
+    const uint2 ch = chunks[blockIdx.x];
+    const Seg s = segs[ch.x];
+    const uint32_t end = chunk_end(ch, s);
+    float clo[3], cext[3];
+    for (int a = 0; a < 3; ++a) {
+        clo[a] = fval(acc[ch.x].key[6 + a]);
+        cext[a] = fval(acc[ch.x].key[9 + a]) - clo[a];
+    }
+    for (int i = threadIdx.x; i < kBinWords; i += kBlock) bins[i] = bin_init_word(i);
+    __syncthreads();
+    for (uint32_t i = ch.y + threadIdx.x; i < end; i += kBlock) {
+        const uint32_t p = order[i];
+        float lo[3], hi[3];
+        load_box(boxes, p, lo, hi);
+        const float4 c4 = centers[p];
+        const float c[3] = {c4.x, c4.y, c4.z};
+        for (int a = 0; a < 3; ++a) {
+            if (cext[a] <= 0) continue;
+            uint32_t* w = bins + 7 * (a * kBins + bin_of(c[a], clo[a], cext[a]));
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(w + k, fkey(lo[k]));
+                atomicMax(w + 3 + k, fkey(hi[k]));
+            }
+            atomicAdd(w + 6, 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBinWords; i += kBlock) {
+        const uint32_t v = bins[i];
+        const int k = i % 7;
+        uint32_t* g = &acc[ch.x].bins[i];
+        if (k < 3) {
+            if (v != KEY_POS_INF) atomicMin(g, v);
+        } else if (k < 6) {
+            if (v != KEY_NEG_INF) atomicMax(g, v);
+        } else if (v) {
+            atomicAdd(g, v);
+        }
+    }
+}
+
+// bvh.hpp:126-191 per segment: the node's box, the leaf test, the SAH split
+// and the child pair.
+__global__ __launch_bounds__(64) void k_decide(Seg* segs, const SegAcc* __restrict__ acc, uint32_t nseg, BNode* nodes,
+                                               uint32_t* depth_of, uint32_t* counters) {
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= nseg) return;
+    Seg s = segs[i];
+    const SegAcc& A = acc[i];
+    FBox box, cen;
+    for (int a = 0; a < 3; ++a) {
+        box.lo[a] = fval(A.key[a]);
+        box.hi[a] = fval(A.key[3 + a]);
+        cen.lo[a] = fval(A.key[6 + a]);
+        cen.hi[a] = fval(A.key[9 + a]);
+    }
+    BNode& nd = nodes[s.node];
+    for (int a = 0; a < 3; ++a) {
+        nd.lo[a] = box.lo[a];
+        nd.hi[a] = box.hi[a];
+    }
+    depth_of[s.node] = s.depth;
+    atomicMax(&counters[1], s.depth);
+    s.kind = 0;
+    bool leaf = s.count == 1 || s.depth + 1 >= kMaxDepth;
+    float best_cost = __builtin_inff();
+    int best_axis = -1, best_bin = 0;
+    if (!leaf) {
+        const float parent = half_area(box);
+        for (int axis = 0; axis < 3; ++axis) {
+            const float lo = cen.lo[axis], extent = cen.hi[axis] - lo;
+            if (extent <= 0) continue;
+            sweep_axis(A.bins + 7 * kBins * axis, parent, axis, best_cost, best_axis, best_bin);
+        }
+        leaf = s.count <= kMaxLeaf && !(best_cost < s.count * kIntersectionCost);
+    }
+    if (leaf) {
+        nd.first = s.first;
+        nd.count = s.count;
+        segs[i] = s;
+        return;
+    }
+    if (best_axis >= 0) {
+        s.kind = 1;
+        s.axis = best_axis;
+        s.bin = best_bin;
+        s.lo = cen.lo[best_axis];
+        s.extent = cen.hi[best_axis] - s.lo;
+        uint32_t l = 0;
+        for (int b = 0; b <= best_bin; ++b) l += A.bins[7 * (best_axis * kBins + b) + 6];
+        s.mid = s.first + l;
+    } else {
+        s.kind = 2;
+        s.mid = s.first + s.count / 2;
+    }
+    s.left = atomicAdd(&counters[0], 2u);
+    nd.first = s.left;
+    nd.count = 0;
+    segs[i] = s;
+}
+
+__device__ __forceinline__ bool goes_left(const float4* __restrict__ centers, uint32_t p, const Seg& s) {
+    const float* c = reinterpret_cast<const float*>(centers + p);
+    return bin_of(c[s.axis], s.lo, s.extent) <= s.bin;
+}
+
+// Block-wide sums of two counters, returned to every thread.
+__device__ __forceinline__ void block_sum2(uint32_t& a, uint32_t& b, uint32_t (*part)[2]) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        part[wave][0] = a;
+        part[wave][1] = b;
+    }
+    __syncthreads();
+    a = b = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        a += part[w][0];
+        b += part[w][1];
+    }
+}
+
+// Per chunk: elements left of the partition point that belong right, and
+// elements right of it that belong left.
+__global__ __launch_bounds__(kBlock) void k_part_count(const float4* __restrict__ centers,
+                                                        const uint32_t* __restrict__ order,
+                                                        const uint2* __restrict__ chunks, const Seg* __restrict__ segs,
+                                                        uint32_t* cnt) {
+    __shared__ uint32_t part[kBlock / 64][2];
+    const uint2 ch = chunks[blockIdx.x];
+    const Seg s = segs[ch.x];
+    if (s.kind != 1) return;
+    const uint32_t end = chunk_end(ch, s);
+    uint32_t nl = 0, nr = 0;
+    for (uint32_t i = ch.y + threadIdx.x; i < end; i += kBlock) {
+        const bool left = goes_left(centers, order[i], s);
+        nl += (i < s.mid && !left) ? 1u : 0u;
+        nr += (i >= s.mid && left) ? 1u : 0u;
+    }
+    block_sum2(nl, nr, part);
+    if (threadIdx.x == 0) {
+        cnt[2 * blockIdx.x] = nl;
+        cnt[2 * blockIdx.x + 1] = nr;
+    }
+}
+
+// Positions of the misplaced elements in scan order: those left of the
+// partition point from the segment's start, those right of it from its end.
+__global__ __launch_bounds__(kBlock) void k_part_pos(const float4* __restrict__ centers,
+                                                      const uint32_t* __restrict__ order,
+                                                      const uint2* __restrict__ chunks, const Seg* __restrict__ segs,
+                                                      const uint32_t* __restrict__ cnt, uint32_t* pos) {
+    __shared__ uint32_t part[kBlock / 64][2];
+    const uint2 ch = chunks[blockIdx.x];
+    const Seg s = segs[ch.x];
+    if (s.kind != 1) return;
+    const uint32_t end = chunk_end(ch, s);
+    uint32_t ol = 0, orr = 0;
+    for (uint32_t c = s.chunk_begin + threadIdx.x; c < blockIdx.x; c += kBlock) {
+        ol += cnt[2 * c];
+        orr += cnt[2 * c + 1];
+    }
+    block_sum2(ol, orr, part);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1;
+    for (uint32_t base = ch.y; base < end; base += kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool lf = false, rt = false;
+        if (i < end) {
+            const bool left = goes_left(centers, order[i], s);
+            lf = i < s.mid && !left;
+            rt = i >= s.mid && left;
+        }
+        const unsigned long long ml = __ballot(lf), mr = __ballot(rt);
+        __syncthreads();
+        if (lane == 0) {
+            part[wave][0] = __popcll(ml);
+            part[wave][1] = __popcll(mr);
+        }
+        __syncthreads();
+        uint32_t bl = 0, br = 0, tl = 0, tr = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            if (w < wave) {
+                bl += part[w][0];
+                br += part[w][1];
+            }
+            tl += part[w][0];
+            tr += part[w][1];
+        }
+        if (lf) pos[s.first + ol + bl + __popcll(ml & below)] = i;
+        if (rt) pos[s.first + s.count - 1 - (orr + br + __popcll(mr & below))] = i;
+        ol += tl;
+        orr += tr;
+    }
+}
+
+// The swaps: the r-th misplaced element from the left with the r-th from the
+// right.
+__global__ __launch_bounds__(kBlock) void k_part_swap(const uint2* __restrict__ chunks, const Seg* __restrict__ segs,
+                                                       const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ pos,
+                                                       uint32_t* order) {
+    __shared__ uint32_t part[kBlock / 64][2];
+    const uint2 ch = chunks[blockIdx.x];
+    const Seg s = segs[ch.x];
+    if (s.kind != 1) return;
+    uint32_t total = 0, unused = 0;
+    for (uint32_t c = s.chunk_begin + threadIdx.x; c < s.chunk_begin + s.chunk_count; c += kBlock) total += cnt[2 * c];
+    block_sum2(total, unused, part);
+    const uint32_t r0 = ch.y - s.first;
+    const uint32_t r1 = min(r0 + kChunk, total);
+    for (uint32_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        const uint32_t a = pos[s.first + r], b = pos[s.first + s.count - total + r];
+        const uint32_t x = order[a];
+        order[a] = order[b];
+        order[b] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Segments of at most kSmall primitives: one wave builds the whole subtree,
+// depth first (bvh.hpp:126-197), on LDS copies of its boxes and centroids.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, const float4* __restrict__ centers,
+                                              uint32_t* order, const uint4* __restrict__ small, BNode* nodes,
+                                              uint32_t* depth_of, uint32_t* counters) {
+    __shared__ float s_lo[3][kSmall], s_hi[3][kSmall], s_c[3][kSmall];
+    __shared__ uint32_t s_order[kSmall];
+    __shared__ uint16_t s_perm[kSmall], s_lst[kSmall];
+    __shared__ uint32_t s_bins[kBinWords];
+    __shared__ uint32_t s_stack[4][kMaxDepth + 8];   // first, count, depth, node
+
+    const uint4 sg = small[blockIdx.x];
+    const uint32_t g0 = sg.x, total = sg.y;
+    const int lane = threadIdx.x;
+    for (uint32_t i = lane; i < total; i += 64) {
+        const uint32_t p = order[g0 + i];
+        float lo[3], hi[3];
+        load_box(boxes, p, lo, hi);
+        const float4 c = centers[p];
+        s_order[i] = p;
+        for (int a = 0; a < 3; ++a) {
+            s_lo[a][i] = lo[a];
+            s_hi[a][i] = hi[a];
+        }
+        s_c[0][i] = c.x;
+        s_c[1][i] = c.y;
+        s_c[2][i] = c.z;
+        s_perm[i] = static_cast<uint16_t>(i);
+    }
+    if (lane == 0) {
+        s_stack[0][0] = 0;
+        s_stack[1][0] = total;
+        s_stack[2][0] = sg.z;
+        s_stack[3][0] = sg.w;
+    }
+    uint32_t sp = 1, deepest = 0;
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1;
+    while (sp > 0) {
+        --sp;
+        const uint32_t f = s_stack[0][sp], c = s_stack[1][sp], d = s_stack[2][sp], node = s_stack[3][sp];
+        __syncthreads();
+        deepest = max(deepest, d);
+        float v[12];
+        bounds_init(v);
+        for (uint32_t k = f + lane; k < f + c; k += 64) {
+            const uint32_t li = s_perm[k];
+            const float lo[3] = {s_lo[0][li], s_lo[1][li], s_lo[2][li]};
+            const float hi[3] = {s_hi[0][li], s_hi[1][li], s_hi[2][li]};
+            const float cc[3] = {s_c[0][li], s_c[1][li], s_c[2][li]};
+            bounds_add(v, lo, hi, cc);
+        }
+        bounds_wave(v);
+        if (lane == 0) {
+            for (int a = 0; a < 3; ++a) {
+                nodes[node].lo[a] = v[a];
+                nodes[node].hi[a] = v[3 + a];
+            }
+            depth_of[node] = d;
+        }
+        if (c == 1 || d + 1 >= kMaxDepth) {
+            if (lane == 0) {
+                nodes[node].first = g0 + f;
+                nodes[node].count = c;
+            }
+            continue;
+        }
+        float clo[3], cext[3];
+        for (int a = 0; a < 3; ++a) {
+            clo[a] = v[6 + a];
+            cext[a] = v[9 + a] - clo[a];
+        }
+        for (int i = lane; i < kBinWords; i += 64) s_bins[i] = bin_init_word(i);
+        __syncthreads();
+        for (uint32_t k = f + lane; k < f + c; k += 64) {
+            const uint32_t li = s_perm[k];
+            for (int a = 0; a < 3; ++a) {
+                if (cext[a] <= 0) continue;
+                uint32_t* w = s_bins + 7 * (a * kBins + bin_of(s_c[a][li], clo[a], cext[a]));
+                for (int q = 0; q < 3; ++q) {
+                    atomicMin(w + q, fkey(s_lo[q][li]));
+                    atomicMax(w + 3 + q, fkey(s_hi[q][li]));
+                }
+                atomicAdd(w + 6, 1u);
+            }
+        }
+        __syncthreads();
+        // Lanes 0-2 sweep one axis each; the first strictly cheapest wins,
+        // axes in order, as the reference's single loop over axes decides.
+        FBox box;
+        for (int a = 0; a < 3; ++a) {
+            box.lo[a] = v[a];
+            box.hi[a] = v[3 + a];
+        }
+        float axis_cost = __builtin_inff();
+        int axis_bin = -1;
+        if (lane < 3) {
+            float cext_l = lane == 0 ? cext[0] : lane == 1 ? cext[1] : cext[2];
+            if (!(cext_l <= 0)) {
+                int ba = -1, bb = 0;
+                sweep_axis(s_bins + 7 * kBins * lane, half_area(box), lane, axis_cost, ba, bb);
+                axis_bin = ba >= 0 ? bb : -1;
+            }
+        }
+        float best_cost = __builtin_inff();
+        int best_axis = -1, best_bin = 0;
+        for (int a = 0; a < 3; ++a) {
+            const float ca = __shfl(axis_cost, a, 64);
+            const int ba = __shfl(axis_bin, a, 64);
+            if (ba >= 0 && ca < best_cost) {
+                best_cost = ca;
+                best_axis = a;
+                best_bin = ba;
+            }
+        }
+        if (c <= kMaxLeaf && !(best_cost < c * kIntersectionCost)) {
+            if (lane == 0) {
+                nodes[node].first = g0 + f;
+                nodes[node].count = c;
+            }
+            continue;
+        }
+        uint32_t mid;
+        if (best_axis >= 0) {
+            uint32_t l = 0;
+            for (int b = 0; b <= best_bin; ++b) l += s_bins[7 * (best_axis * kBins + b) + 6];
+            mid = f + l;
+            const float lo = clo[best_axis], ext = cext[best_axis];
+            uint32_t nl = 0, nr = 0;
+            for (uint32_t base = f; base < f + c; base += 64) {
+                const uint32_t k = base + lane;
+                bool lf = false, rt = false;
+                if (k < f + c) {
+                    const bool left = bin_of(s_c[best_axis][s_perm[k]], lo, ext) <= best_bin;
+                    lf = k < mid && !left;
+                    rt = k >= mid && left;
+                }
+                const unsigned long long ml = __ballot(lf), mr = __ballot(rt);
+                if (lf) s_lst[f + nl + __popcll(ml & below)] = static_cast<uint16_t>(k);
+                if (rt) s_lst[f + c - 1 - (nr + __popcll(mr & below))] = static_cast<uint16_t>(k);
+                nl += __popcll(ml);
+                nr += __popcll(mr);
+            }
+            __syncthreads();
+            for (uint32_t r = lane; r < nl; r += 64) {
+                const uint32_t a = s_lst[f + r], b = s_lst[f + c - nl + r];
+                const uint16_t x = s_perm[a];
+                s_perm[a] = s_perm[b];
+                s_perm[b] = x;
+            }
+            __syncthreads();
+        } else {
+            mid = f + c / 2;
+        }
+        if (lane == 0) {
+            const uint32_t left = atomicAdd(&counters[0], 2u);
+            nodes[node].first = left;
+            nodes[node].count = 0;
+            s_stack[0][sp] = mid;
+            s_stack[1][sp] = f + c - mid;
+            s_stack[2][sp] = d + 1;
+            s_stack[3][sp] = left + 1;
+            s_stack[0][sp + 1] = f;
+            s_stack[1][sp + 1] = mid - f;
+            s_stack[2][sp + 1] = d + 1;
+            s_stack[3][sp + 1] = left;
+        }
+        sp += 2;
+        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < total; i += 64) order[g0 + i] = s_order[s_perm[i]];
+    if (lane == 0) atomicMax(&counters[1], deepest);
+}
+
+// ---------------------------------------------------------------------------
+// 4-wide collapse (bvh.hpp:205-242).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ FBox node_box(const BNode& b) {
+    FBox x;
+    for (int a = 0; a < 3; ++a) {
+        x.lo[a] = b.lo[a];
+        x.hi[a] = b.hi[a];
+    }
+    return x;
+}
+
+// Every node's child list as collapse(b) would open it: the interior child
+// of largest area (the first of equals) until four or only leaves remain.
+__global__ __launch_bounds__(kBlock) void k_clist(const BNode* __restrict__ nodes, uint32_t total, int4* clist) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= total) return;
+    int child[4] = {static_cast<int>(b), -1, -1, -1};
+    int n = 1;
+    while (n < 4) {
+        int widest = -1;
+        float widest_area = -1;
+        for (int i = 0; i < n; ++i) {
+            const BNode c = nodes[child[i]];
+            if (c.count != 0) continue;
+            const float area = half_area(node_box(c));
+            if (area > widest_area) {
+                widest = i;
+                widest_area = area;
+            }
+        }
+        if (widest < 0) break;
+        const uint32_t left = nodes[child[widest]].first;
+        child[widest] = static_cast<int>(left);
+        child[n++] = static_cast<int>(left + 1);
+    }
+    clist[b] = make_int4(child[0], child[1], child[2], child[3]);
+}
+
+__device__ __forceinline__ int child_at(const int4& c, int i) {
+    return i == 0 ? c.x : i == 1 ? c.y : i == 2 ? c.z : c.w;
+}
+
+// Collapsed subtree sizes of the interior nodes at binary depth d (their
+// collapsed children are deeper, so already sized).
+__global__ __launch_bounds__(kBlock) void k_wsize(const BNode* __restrict__ nodes, const uint32_t* __restrict__ depth_of,
+                                                   const int4* __restrict__ clist, uint32_t total, uint32_t d,
+                                                   uint32_t* wsize) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= total || depth_of[b] != d || nodes[b].count != 0) return;
+    const int4 cl = clist[b];
+    uint32_t w = 1;
+    for (int i = 0; i < 4; ++i) {
+        const int c = child_at(cl, i);
+        if (c >= 0 && nodes[c].count == 0) w += wsize[c];
+    }
+    wsize[b] = w;
+}
+
+// Preorder numbers (collapse's emplace order) of the collapsed children of
+// the numbered interior nodes at binary depth d.
+__global__ __launch_bounds__(kBlock) void k_index(const BNode* __restrict__ nodes, const uint32_t* __restrict__ depth_of,
+                                                   const int4* __restrict__ clist, uint32_t total, uint32_t d,
+                                                   const uint32_t* __restrict__ wsize, int32_t* index) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= total || depth_of[b] != d || nodes[b].count != 0) return;
+    const int32_t at = index[b];
+    if (at < 0) return;
+    const int4 cl = clist[b];
+    uint32_t next = static_cast<uint32_t>(at) + 1;
+    for (int i = 0; i < 4; ++i) {
+        const int c = child_at(cl, i);
+        if (c < 0 || nodes[c].count != 0) continue;
+        index[c] = static_cast<int32_t>(next);
+        next += wsize[c];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_index_init(uint32_t total, int32_t* index) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b < total) index[b] = b == 0 ? 0 : -1;
+}
+
+// The 4-wide nodes (bvh.hpp:222-241): empty slots are +inf boxes with
+// first = count = 0.
+__global__ __launch_bounds__(kBlock) void k_emit(const BNode* __restrict__ nodes, const int4* __restrict__ clist,
+                                                  const int32_t* __restrict__ index, uint32_t total, WNode* out) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= total) return;
+    const int32_t at = index[b];
+    if (at < 0) return;
+    const int4 cl = clist[b];
+    WNode w;
+    const float inf = __builtin_inff();
+    for (int i = 0; i < 4; ++i) {
+        const int c = child_at(cl, i);
+        if (c < 0) {
+            for (int a = 0; a < 3; ++a) {
+                w.lo[a][i] = inf;
+                w.hi[a][i] = inf;
+            }
+            w.first[i] = 0;
+            w.count[i] = 0;
+            continue;
+        }
+        const BNode n = nodes[c];
+        for (int a = 0; a < 3; ++a) {
+            w.lo[a][i] = n.lo[a];
+            w.hi[a][i] = n.hi[a];
+        }
+        if (n.count > 0) {
+            w.first[i] = n.first;
+            w.count[i] = n.count;
+        } else {
+            w.first[i] = static_cast<uint32_t>(index[c]);
+            w.count[i] = 0;
+        }
+    }
+    out[at] = w;
+}
+
+struct DevMem {
+    std::vector<void*> ptrs;
+    ~DevMem() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    hipError_t get(T** p, size_t count) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 16);
+        if (e != hipSuccess) return e;
+        ptrs.push_back(q);
+        *p = static_cast<T*>(q);
+        return hipSuccess;
+    }
+};
+
+#define BVH_TRY(x)                              \
+    do {                                        \
+        const hipError_t e_ = (x);              \
+        if (e_ != hipSuccess) return e_;        \
+    } while (0)
+
+inline uint32_t blocks(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_nodes, uint32_t* n_nodes,
+                 hipStream_t st) {
+    if (n == 0) return hipErrorInvalidValue;
+    const uint32_t cap = 2 * n;   // binary nodes: at most 2n - 1
+    DevMem mem;
+    float4* centers;
+    uint32_t *pos, *depth_of, *counters, *wsize, *cnt;
+    BNode* nodes;
+    int4* clist;
+    int32_t* index;
+    BVH_TRY(mem.get(&centers, n));
+    BVH_TRY(mem.get(&pos, n));
+    BVH_TRY(mem.get(&nodes, cap));
+    BVH_TRY(mem.get(&depth_of, cap));
+    BVH_TRY(mem.get(&clist, cap));
+    BVH_TRY(mem.get(&wsize, cap));
+    BVH_TRY(mem.get(&index, cap));
+    BVH_TRY(mem.get(&counters, 4));
+    const uint32_t max_segs = n / kSmall + 2;
+    const uint32_t max_chunks = n / kChunk + max_segs + 2;
+    Seg* segs;
+    SegAcc* acc;
+    uint2* chunks;
+    BVH_TRY(mem.get(&segs, max_segs));
+    BVH_TRY(mem.get(&acc, max_segs));
+    BVH_TRY(mem.get(&chunks, max_chunks));
+    BVH_TRY(mem.get(&cnt, 2 * max_chunks));
+
+    k_setup<<<blocks(n), kBlock, 0, st>>>(d_boxes, n, centers, d_order, counters);
+    BVH_TRY(hipGetLastError());
+
+    std::vector<uint4> small;
+    std::vector<Seg> cur;
+    auto add = [&](std::vector<Seg>& next, uint32_t first, uint32_t count, uint32_t depth, uint32_t node) {
+        if (count > kSmall) {
+            Seg s{};
+            s.first = first;
+            s.count = count;
+            s.depth = depth;
+            s.node = node;
+            next.push_back(s);
+        } else {
+            small.push_back(make_uint4(first, count, depth, node));
+        }
+    };
+    add(cur, 0, n, 0, 0);
+    std::vector<uint2> chunk_list;
+    while (!cur.empty()) {
+        const uint32_t nseg = static_cast<uint32_t>(cur.size());
+        if (nseg > max_segs) return hipErrorInvalidValue;   // cannot happen: segments are disjoint and > kSmall
+        chunk_list.clear();
+        for (uint32_t i = 0; i < nseg; ++i) {
+            cur[i].chunk_begin = static_cast<uint32_t>(chunk_list.size());
+            for (uint32_t p = cur[i].first; p < cur[i].first + cur[i].count; p += kChunk) chunk_list.push_back(make_uint2(i, p));
+            cur[i].chunk_count = static_cast<uint32_t>(chunk_list.size()) - cur[i].chunk_begin;
+        }
+        const uint32_t nch = static_cast<uint32_t>(chunk_list.size());
+        if (nch > max_chunks) return hipErrorInvalidValue;
+        BVH_TRY(hipMemcpyAsync(segs, cur.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice, st));
+        BVH_TRY(hipMemcpyAsync(chunks, chunk_list.data(), nch * sizeof(uint2), hipMemcpyHostToDevice, st));
+        k_seg_init<<<nseg, kBlock, 0, st>>>(acc);
+        k_bounds<<<nch, kBlock, 0, st>>>(d_boxes, centers, d_order, chunks, segs, acc);
+        k_bin<<<nch, kBlock, 0, st>>>(d_boxes, centers, d_order, chunks, segs, acc);
+        k_decide<<<(nseg + 63) / 64, 64, 0, st>>>(segs, acc, nseg, nodes, depth_of, counters);
+        k_part_count<<<nch, kBlock, 0, st>>>(centers, d_order, chunks, segs, cnt);
+        k_part_pos<<<nch, kBlock, 0, st>>>(centers, d_order, chunks, segs, cnt, pos);
+        k_part_swap<<<nch, kBlock, 0, st>>>(chunks, segs, cnt, pos, d_order);
+        BVH_TRY(hipGetLastError());
+        BVH_TRY(hipMemcpyAsync(cur.data(), segs, nseg * sizeof(Seg), hipMemcpyDeviceToHost, st));
+        BVH_TRY(hipStreamSynchronize(st));
+        std::vector<Seg> next;
+        for (const Seg& s : cur) {
+            if (s.kind == 0) continue;
+            add(next, s.first, s.mid - s.first, s.depth + 1, s.left);
+            add(next, s.mid, s.first + s.count - s.mid, s.depth + 1, s.left + 1);
+        }
+        cur.swap(next);
+    }
+
+    if (!small.empty()) {
+        uint4* d_small;
+        BVH_TRY(mem.get(&d_small, small.size()));
+        BVH_TRY(hipMemcpyAsync(d_small, small.data(), small.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+        k_small<<<static_cast<uint32_t>(small.size()), 64, 0, st>>>(d_boxes, centers, d_order, d_small, nodes, depth_of,
+                                                                    counters);
+        BVH_TRY(hipGetLastError());
+    }
+    uint32_t hc[2];
+    BVH_TRY(hipMemcpyAsync(hc, counters, 8, hipMemcpyDeviceToHost, st));
+    BVH_TRY(hipStreamSynchronize(st));
+    const uint32_t total = hc[0], deepest = hc[1];
+    if (total > cap || deepest >= kMaxDepth) return hipErrorInvalidValue;
+
+    k_clist<<<blocks(total), kBlock, 0, st>>>(nodes, total, clist);
+    k_index_init<<<blocks(total), kBlock, 0, st>>>(total, index);
+    for (int d = static_cast<int>(deepest); d >= 0; --d)
+        k_wsize<<<blocks(total), kBlock, 0, st>>>(nodes, depth_of, clist, total, static_cast<uint32_t>(d), wsize);
+    for (uint32_t d = 0; d <= deepest; ++d)
+        k_index<<<blocks(total), kBlock, 0, st>>>(nodes, depth_of, clist, total, d, wsize, index);
+    k_emit<<<blocks(total), kBlock, 0, st>>>(nodes, clist, index, total, static_cast<WNode*>(d_nodes));
+    BVH_TRY(hipGetLastError());
+    uint32_t root_count = 0, root_size = 0;
+    BVH_TRY(hipMemcpyAsync(&root_count, &nodes[0].count, 4, hipMemcpyDeviceToHost, st));
+    BVH_TRY(hipMemcpyAsync(&root_size, wsize, 4, hipMemcpyDeviceToHost, st));
+    BVH_TRY(hipStreamSynchronize(st));
+    *n_nodes = root_count > 0 ? 1 : root_size;
+    return hipSuccess;
+}
+
+}  // namespace bvhg
+}  // namespace ptd

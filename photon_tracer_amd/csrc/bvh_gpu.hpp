@@ -1,0 +1,22 @@
+// Device BVH builder: the reference's binned-SAH build and 4-wide collapse
+// (bvh.hpp:39-52,126-247) as gfx950 kernels, producing the same nodes and
+// the same primitive order as the host builder (bvh_build.cpp), word for word.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ptd {
+namespace bvhg {
+
+// Builds the tree over n boxes. d_boxes: n boxes on the device, 6 floats
+// each (min xyz, max xyz: the aabb layout, aabb.hpp:9). Outputs on the
+// device: d_order (n words) and d_nodes (128-byte 4-wide nodes, room for
+// max(n, 1)); *n_nodes is the number written. n >= 1. Allocates its own
+// scratch and synchronises `stream` before returning.
+hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_nodes, uint32_t* n_nodes,
+                 hipStream_t stream);
+
+}  // namespace bvhg
+}  // namespace ptd

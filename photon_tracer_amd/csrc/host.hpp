@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -168,8 +169,14 @@ int tree_depth(const std::vector<bvh_node>& nodes, uint32_t n);
 box3 sphere_box(const sphere_rec& s);
 box3 triangle_box(const triangle_rec& t);
 
-// PTSD -> PTGP v1 words (geometry::build + pack_for_gpu + serialize).
-std::vector<uint32_t> pack_ptgp(const scene_desc& s);
+// PTSD -> PTGP v1 words (geometry::build + pack_for_gpu + serialize). With
+// a builder, every tree is built by it (one call at a time) instead of
+// build_bvh on host threads; it must return build_bvh's tree.
+using bvh_builder = std::function<bvh_tree(const std::vector<box3>&)>;
+std::vector<uint32_t> pack_ptgp(const scene_desc& s, const bvh_builder* builder = nullptr);
+
+// A malloc'd copy of v for the C ABI (free with pt_free); *n = v.size().
+uint32_t* copy_out(const std::vector<uint32_t>& v, size_t* n);
 
 // Batched renders (merge.cpp): several flat PTGP v1 scenes (no instances) as
 // one, every array concatenated with its indices rebased and a few 4-wide

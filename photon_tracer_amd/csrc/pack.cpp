@@ -270,7 +270,8 @@ uint32_t append_tree(std::vector<bvh_node>& nodes, std::vector<uint32_t>& order,
 
 }  // namespace
 
-std::vector<uint32_t> pack_ptgp(const scene_desc& s) {
+std::vector<uint32_t> pack_ptgp(const scene_desc& s, const bvh_builder* builder) {
+    const auto build = [&](const std::vector<box3>& boxes) { return builder ? (*builder)(boxes) : build_bvh(boxes); };
     // geometry::build (geometry.hpp:74-104): lights, mesh trees, flat tree, instance tree.
     std::vector<uint32_t> lights;
     for (uint32_t i = 0; i < s.spheres.size(); ++i)
@@ -279,10 +280,11 @@ std::vector<uint32_t> pack_ptgp(const scene_desc& s) {
     std::vector<bvh_tree> mesh_trees(s.meshes.size());
     std::vector<box3> mesh_bounds(s.meshes.size());
     {
-        // Meshes are independent: build them side by side.
+        // Meshes are independent: build them side by side on host threads,
+        // or one after another with a given builder.
         std::vector<std::thread> pool;
         for (size_t k = 0; k < s.meshes.size(); ++k) {
-            pool.emplace_back([&, k] {
+            const auto one = [&, k] {
                 std::vector<box3> boxes;
                 boxes.reserve(s.meshes[k].size());
                 box3 bounds;
@@ -291,8 +293,13 @@ std::vector<uint32_t> pack_ptgp(const scene_desc& s) {
                     bounds.grow(boxes.back());
                 }
                 mesh_bounds[k] = bounds;
-                mesh_trees[k] = build_bvh(boxes);
-            });
+                mesh_trees[k] = build(boxes);
+            };
+            if (builder) {
+                one();
+                continue;
+            }
+            pool.emplace_back(one);
             if (pool.size() >= 8) {
                 for (auto& t : pool) t.join();
                 pool.clear();
@@ -306,10 +313,10 @@ std::vector<uint32_t> pack_ptgp(const scene_desc& s) {
         boxes.reserve(s.spheres.size() + s.triangles.size());
         for (const auto& sp : s.spheres) boxes.push_back(sphere_box(sp));
         for (const auto& t : s.triangles) boxes.push_back(triangle_box(t));
-        flat = build_bvh(boxes);
+        flat = build(boxes);
         boxes.clear();
         for (const auto& in : s.instances) boxes.push_back(in.to_world.box(mesh_bounds[in.mesh]));
-        inst_tree = build_bvh(boxes);
+        inst_tree = build(boxes);
     }
 
     // pack_for_gpu (gpu_pack.hpp:84-175)

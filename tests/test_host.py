@@ -101,6 +101,45 @@ def test_bad_buffers_are_refused():
         pt.generate(9, 0, 1.0)
 
 
+def test_bvh_build_host_path():
+    # pt_bvh_build without PT_BVH_DEVICE runs the host builder that the pack
+    # test above pins to the reference; here the C ABI around it: every
+    # primitive in exactly one leaf, children inside their parents' slots,
+    # empty slots +inf, and argument errors.
+    rng = np.random.default_rng(2)
+    c = rng.uniform(-5, 5, (3000, 3))
+    boxes = np.concatenate([c - 0.05, c + 0.05], 1).astype(np.float32)
+    nodes, order, _ = pt.bvh_build(boxes)
+    lo = nodes[:, 0:12].view(np.float32).reshape(-1, 3, 4)
+    hi = nodes[:, 12:24].view(np.float32).reshape(-1, 3, 4)
+    first, count = nodes[:, 24:28], nodes[:, 28:32]
+    seen = []
+    for k in range(len(nodes)):
+        for s in range(4):
+            if lo[k, 0, s] == np.inf:
+                assert hi[k, 0, s] == np.inf and first[k, s] == 0 and count[k, s] == 0
+                continue
+            if count[k, s]:
+                prims = order[first[k, s]:first[k, s] + count[k, s]]
+                seen.append(prims)
+                assert (boxes[prims, :3] >= lo[k, :, s]).all() and (boxes[prims, 3:] <= hi[k, :, s]).all()
+            else:
+                assert first[k, s] > k   # preorder: children after their parent
+    assert np.array_equal(np.sort(np.concatenate(seen)), np.arange(3000))
+    again = pt.bvh_build(boxes)
+    assert np.array_equal(again[0], nodes) and np.array_equal(again[1], order)
+    empty = pt.bvh_build(np.zeros((0, 6), np.float32))
+    assert empty[0].shape == (0, 32) and empty[1].size == 0
+    lib = pt.library()
+    n_nodes = ctypes.c_size_t(0)
+    p1, p2 = ctypes.POINTER(ctypes.c_uint32)(), ctypes.POINTER(ctypes.c_uint32)()
+    assert lib.pt_bvh_build(boxes.ctypes.data, 3000, 4, ctypes.byref(p1), ctypes.byref(n_nodes), ctypes.byref(p2),
+                            None) != 0
+    assert lib.pt_bvh_build(None, 3, 0, ctypes.byref(p1), ctypes.byref(n_nodes), ctypes.byref(p2), None) != 0
+    w = pt.generate(0, 0, 0.25)
+    assert lib.pt_pack_ex(w.ctypes.data, w.size, 2, ctypes.byref(p1), ctypes.byref(n_nodes)) != 0
+
+
 def test_cycle_is_refused():
     ptgp = pt.pack(pt.generate(0, 0, 0.25))
     bad = ptgp.copy()
