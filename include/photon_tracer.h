@@ -100,8 +100,9 @@ pt_status pt_pack_ex(const uint32_t* ptsd, size_t n_words, int flags, uint32_t**
  * allocated by the library (free with pt_free); n = 0 gives no nodes.
  * flags: PT_BVH_DEVICE builds on the GPU (level-parallel binning and
  * partition, one wave per small subtree, parallel collapse) with the same
- * result word for word; 0 builds on host threads. ms: wall-clock of the call,
- * transfers included (may be NULL). At most 2^30 boxes. */
+ * result word for word; 0 builds on host threads. ms: wall-clock of the
+ * build, the device's transfers and allocations included, the copies into
+ * the returned arrays not (may be NULL). At most 2^30 boxes. */
 enum { PT_BVH_DEVICE = 1 };
 pt_status pt_bvh_build(const float* boxes, size_t n, int flags, uint32_t** nodes, size_t* n_nodes, uint32_t** order,
                        double* ms);

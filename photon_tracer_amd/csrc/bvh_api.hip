@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "api_internal.hpp"
 #include "bvh_gpu.hpp"
@@ -17,10 +18,17 @@ namespace {
 
 constexpr size_t kMaxBoxes = size_t(1) << 30;
 
-struct owned {
-    dbuf b;
-    ~owned() { b.release(); }
+// Device buffers kept between builds (grown, never shrunk, until the
+// process ends): allocating a few hundred MB per call cost more than the
+// build of 1M boxes itself. One build at a time per process.
+struct Pool {
+    std::mutex m;
+    dbuf boxes, order, nodes, scratch;
 };
+Pool& pool() {
+    static Pool* p = new Pool;   // never destroyed: no device frees after the runtime is gone
+    return *p;
+}
 
 }  // namespace
 
@@ -31,18 +39,22 @@ pt::bvh_tree device_bvh(const std::vector<pt::box3>& boxes) {
     const size_t n = boxes.size();
     if (n == 0) return out;
     if (n > kMaxBoxes) throw pt::status_error(PT_ERR_LIMIT, "BVH build: more than 2^30 primitives");
-    owned b, order, nodes;
-    b.b.ensure(24 * n);
-    order.b.ensure(4 * n);
-    nodes.b.ensure(sizeof(pt::bvh_node) * n);
-    HIPC(hipMemcpy(b.b.p, boxes.data(), 24 * n, hipMemcpyHostToDevice));
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lock(P.m);
+    P.boxes.ensure(24 * n);
+    P.order.ensure(4 * n);
+    P.nodes.ensure(sizeof(pt::bvh_node) * n);
+    const size_t scratch = ptd::bvhg::scratch_bytes(static_cast<uint32_t>(n));
+    P.scratch.ensure(scratch);
+    HIPC(hipMemcpy(P.boxes.p, boxes.data(), 24 * n, hipMemcpyHostToDevice));
     uint32_t k = 0;
-    HIPC(ptd::bvhg::build(b.b.as<float>(), static_cast<uint32_t>(n), order.b.as<uint32_t>(), nodes.b.p, &k, nullptr));
+    HIPC(ptd::bvhg::build(P.boxes.as<float>(), static_cast<uint32_t>(n), P.order.as<uint32_t>(), P.nodes.p, &k,
+                          P.scratch.p, scratch, nullptr));
     if (k == 0 || k > n) throw pt::status_error(PT_ERR_DEVICE, "BVH build: bad node count from the device");
     out.nodes.resize(k);
     out.order.resize(n);
-    HIPC(hipMemcpy(out.nodes.data(), nodes.b.p, sizeof(pt::bvh_node) * k, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(out.order.data(), order.b.p, 4 * n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(out.nodes.data(), P.nodes.p, sizeof(pt::bvh_node) * k, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(out.order.data(), P.order.p, 4 * n, hipMemcpyDeviceToHost));
     return out;
 }
 
@@ -60,23 +72,24 @@ pt_status pt_bvh_build(const float* boxes, size_t n, int flags, uint32_t** nodes
     if (flags & ~PT_BVH_DEVICE) return pt::fail(PT_ERR_INVALID, "pt_bvh_build: unknown flags");
     if (n > ptapi::kMaxBoxes) return pt::fail(PT_ERR_LIMIT, "pt_bvh_build: more than 2^30 boxes");
     PT_GUARD({
-        const auto t0 = std::chrono::steady_clock::now();
         std::vector<pt::box3> b(n);
         if (n) std::memcpy(b.data(), boxes, 24 * n);
+        const auto t0 = std::chrono::steady_clock::now();
         const pt::bvh_tree t = (flags & PT_BVH_DEVICE) ? ptapi::device_bvh(b) : pt::build_bvh(b);
-        const std::vector<uint32_t> words(reinterpret_cast<const uint32_t*>(t.nodes.data()),
-                                          reinterpret_cast<const uint32_t*>(t.nodes.data() + t.nodes.size()));
-        size_t n_words = 0, n_order = 0;
-        *nodes = pt::copy_out(words, &n_words);
+        const auto t1 = std::chrono::steady_clock::now();
+        size_t n_order = 0;
+        uint32_t* words = static_cast<uint32_t*>(std::malloc(sizeof(pt::bvh_node) * t.nodes.size() + 4));
+        if (!words) throw std::bad_alloc();
+        if (!t.nodes.empty()) std::memcpy(words, t.nodes.data(), sizeof(pt::bvh_node) * t.nodes.size());
         try {
             *order = pt::copy_out(t.order, &n_order);
         } catch (...) {
-            std::free(*nodes);
-            *nodes = nullptr;
+            std::free(words);
             throw;
         }
+        *nodes = words;
         *n_nodes = t.nodes.size();
-        if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     });
 }
 

@@ -35,7 +35,11 @@ constexpr uint32_t kMaxLeaf = 8;
 constexpr uint32_t kMaxDepth = 64;
 constexpr float kTraversalCost = 1, kIntersectionCost = 1;
 
-constexpr uint32_t kSmall = 1024;   // longest segment one wave finishes in LDS
+// Longest segment one wave finishes in LDS. The wave's work is a chain of
+// dependent LDS steps per node, so what pays is waves per CU: 256 keeps the
+// LDS copy near 14 KB (11 waves per CU); 1024 (46 KB, 3 waves) took 19.6 ms
+// of a 22.6 ms build of 1M boxes.
+constexpr uint32_t kSmall = 256;
 constexpr uint32_t kChunk = 4096;   // primitives per workgroup in the level passes
 constexpr int kBlock = 256;
 constexpr int kBinWords = 3 * kBins * 7;   // per axis and bin: min xyz, max xyz (keys), count
@@ -65,6 +69,14 @@ struct Seg {   // one open segment of a level pass
     float lo, extent;   // centroid bounds along the split axis
     uint32_t pad[3];
 };
+
+struct SmallSeg {   // a segment one wave finishes
+    uint32_t first, count, depth, node;
+    uint32_t base;   // its subtree's nodes: [base, base + 2 * count - 2)
+    uint32_t pad[3];
+};
+
+constexpr uint32_t NO_DEPTH = 0xffffffffu;   // depth_of of an unused node slot
 
 struct SegAcc {
     uint32_t key[12];   // box min, box max, centroid min, centroid max
@@ -221,8 +233,10 @@ __device__ __forceinline__ uint32_t bin_init_word(int i) {
 // Set-up: centroids (aabb::center, aabb.hpp:22), identity order, counters.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_setup(const float* __restrict__ boxes, uint32_t n, float4* centers,
-                                                   uint32_t* order, uint32_t* counters) {
+                                                   uint32_t* order, uint32_t* counters, uint32_t* depth_of,
+                                                   uint32_t cap) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    for (uint32_t k = i; k < cap; k += gridDim.x * kBlock) depth_of[k] = NO_DEPTH;
     if (i == 0) {
         counters[0] = 1;   // node 0 is the root
         counters[1] = 0;   // deepest binary depth
@@ -514,9 +528,12 @@ __global__ __launch_bounds__(kBlock) void k_part_swap(const uint2* __restrict__ 
 // ---------------------------------------------------------------------------
 // Segments of at most kSmall primitives: one wave builds the whole subtree,
 // depth first (bvh.hpp:126-197), on LDS copies of its boxes and centroids.
+// Its new nodes come from its own range (a subtree over c primitives has at
+// most 2c - 2 nodes below its root), not from a shared counter: one global
+// atomic per split serialised the waves.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, const float4* __restrict__ centers,
-                                              uint32_t* order, const uint4* __restrict__ small, BNode* nodes,
+                                              uint32_t* order, const SmallSeg* __restrict__ small, BNode* nodes,
                                               uint32_t* depth_of, uint32_t* counters) {
     __shared__ float s_lo[3][kSmall], s_hi[3][kSmall], s_c[3][kSmall];
     __shared__ uint32_t s_order[kSmall];
@@ -524,8 +541,9 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
     __shared__ uint32_t s_bins[kBinWords];
     __shared__ uint32_t s_stack[4][kMaxDepth + 8];   // first, count, depth, node
 
-    const uint4 sg = small[blockIdx.x];
-    const uint32_t g0 = sg.x, total = sg.y;
+    const SmallSeg sg = small[blockIdx.x];
+    const uint32_t g0 = sg.first, total = sg.count;
+    uint32_t next = sg.base;
     const int lane = threadIdx.x;
     for (uint32_t i = lane; i < total; i += 64) {
         const uint32_t p = order[g0 + i];
@@ -545,8 +563,8 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
     if (lane == 0) {
         s_stack[0][0] = 0;
         s_stack[1][0] = total;
-        s_stack[2][0] = sg.z;
-        s_stack[3][0] = sg.w;
+        s_stack[2][0] = sg.depth;
+        s_stack[3][0] = sg.node;
     }
     uint32_t sp = 1, deepest = 0;
     __syncthreads();
@@ -600,34 +618,81 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
             }
         }
         __syncthreads();
-        // Lanes 0-2 sweep one axis each; the first strictly cheapest wins,
-        // axes in order, as the reference's single loop over axes decides.
+        // The SAH sweep (bvh.hpp:149-170) on 48 lanes, one bin boundary each
+        // (lane = 16 * axis + bin): prefix and suffix boxes and counts by
+        // shuffles within each axis's 16 lanes (min, max and integer sums do
+        // not depend on the order they are taken in), each boundary's cost
+        // by the reference's expression, then the cheapest boundary, the
+        // first of equals in (axis, bin) order: the one the reference's
+        // sequential strict-less scan keeps.
         FBox box;
         for (int a = 0; a < 3; ++a) {
             box.lo[a] = v[a];
             box.hi[a] = v[3 + a];
         }
-        float axis_cost = __builtin_inff();
-        int axis_bin = -1;
-        if (lane < 3) {
-            float cext_l = lane == 0 ? cext[0] : lane == 1 ? cext[1] : cext[2];
-            if (!(cext_l <= 0)) {
-                int ba = -1, bb = 0;
-                sweep_axis(s_bins + 7 * kBins * lane, half_area(box), lane, axis_cost, ba, bb);
-                axis_bin = ba >= 0 ? bb : -1;
+        const int sa = lane >> 4, sb = lane & 15;
+        const float ext_l = sa == 0 ? cext[0] : sa == 1 ? cext[1] : cext[2];
+        const bool axis_ok = lane < 48 && !(ext_l <= 0);
+        FBox pre, suf;
+        uint32_t pc = 0;
+        box_empty(pre);
+        if (lane < 48) {
+            const uint32_t* w = s_bins + 7 * lane;
+            for (int q = 0; q < 3; ++q) {
+                pre.lo[q] = fval(w[q]);
+                pre.hi[q] = fval(w[3 + q]);
+            }
+            pc = w[6];
+        }
+        suf = pre;
+        uint32_t sc = pc;
+#pragma unroll
+        for (int dl = 1; dl < kBins; dl <<= 1) {
+            FBox up, dn;
+            for (int q = 0; q < 3; ++q) {
+                up.lo[q] = __shfl_up(pre.lo[q], dl, kBins);
+                up.hi[q] = __shfl_up(pre.hi[q], dl, kBins);
+                dn.lo[q] = __shfl_down(suf.lo[q], dl, kBins);
+                dn.hi[q] = __shfl_down(suf.hi[q], dl, kBins);
+            }
+            const uint32_t upc = __shfl_up(pc, dl, kBins), dnc = __shfl_down(sc, dl, kBins);
+            if (sb >= dl) {
+                box_grow(pre, up.lo, up.hi);
+                pc += upc;
+            }
+            if (sb + dl < kBins) {
+                box_grow(suf, dn.lo, dn.hi);
+                sc += dnc;
             }
         }
-        float best_cost = __builtin_inff();
-        int best_axis = -1, best_bin = 0;
-        for (int a = 0; a < 3; ++a) {
-            const float ca = __shfl(axis_cost, a, 64);
-            const int ba = __shfl(axis_bin, a, 64);
-            if (ba >= 0 && ca < best_cost) {
-                best_cost = ca;
-                best_axis = a;
-                best_bin = ba;
+        FBox right;
+        for (int q = 0; q < 3; ++q) {
+            right.lo[q] = __shfl_down(suf.lo[q], 1, kBins);
+            right.hi[q] = __shfl_down(suf.hi[q], 1, kBins);
+        }
+        const uint32_t rc = __shfl_down(sc, 1, kBins);
+        float cand_cost = __builtin_inff();
+        int cand = 64;
+        if (axis_ok && sb < kBins - 1 && pc != 0 && rc != 0) {
+            const float cost = kTraversalCost + kIntersectionCost *
+                                                    (half_area(pre) * pc + half_area(right) * rc) / half_area(box);
+            if (cost < __builtin_inff()) {
+                cand_cost = cost;
+                cand = lane;
             }
         }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const float oc = __shfl_xor(cand_cost, m, 64);
+            const int oi = __shfl_xor(cand, m, 64);
+            if (oi < 64 && (cand >= 64 || oc < cand_cost || (oc == cand_cost && oi < cand))) {
+                cand_cost = oc;
+                cand = oi;
+            }
+        }
+        const float best_cost = cand < 64 ? cand_cost : __builtin_inff();
+        const int best_axis = cand < 64 ? cand >> 4 : -1, best_bin = cand & 15;
+        const uint32_t left_count = __shfl(pc, cand < 64 ? cand : 0, 64);
         if (c <= kMaxLeaf && !(best_cost < c * kIntersectionCost)) {
             if (lane == 0) {
                 nodes[node].first = g0 + f;
@@ -637,9 +702,7 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
         }
         uint32_t mid;
         if (best_axis >= 0) {
-            uint32_t l = 0;
-            for (int b = 0; b <= best_bin; ++b) l += s_bins[7 * (best_axis * kBins + b) + 6];
-            mid = f + l;
+            mid = f + left_count;
             const float lo = clo[best_axis], ext = cext[best_axis];
             uint32_t nl = 0, nr = 0;
             for (uint32_t base = f; base < f + c; base += 64) {
@@ -667,8 +730,9 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
         } else {
             mid = f + c / 2;
         }
+        const uint32_t left = next;
+        next += 2;
         if (lane == 0) {
-            const uint32_t left = atomicAdd(&counters[0], 2u);
             nodes[node].first = left;
             nodes[node].count = 0;
             s_stack[0][sp] = mid;
@@ -702,9 +766,10 @@ __device__ __forceinline__ FBox node_box(const BNode& b) {
 
 // Every node's child list as collapse(b) would open it: the interior child
 // of largest area (the first of equals) until four or only leaves remain.
-__global__ __launch_bounds__(kBlock) void k_clist(const BNode* __restrict__ nodes, uint32_t total, int4* clist) {
+__global__ __launch_bounds__(kBlock) void k_clist(const BNode* __restrict__ nodes, const uint32_t* __restrict__ depth_of,
+                                                   uint32_t total, int4* clist) {
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= total) return;
+    if (b >= total || depth_of[b] == NO_DEPTH) return;
     int child[4] = {static_cast<int>(b), -1, -1, -1};
     int n = 1;
     while (n < 4) {
@@ -809,18 +874,18 @@ __global__ __launch_bounds__(kBlock) void k_emit(const BNode* __restrict__ nodes
     out[at] = w;
 }
 
+// The build's scratch, carved from the caller's block in 256-byte aligned
+// pieces.
 struct DevMem {
-    std::vector<void*> ptrs;
-    ~DevMem() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    static size_t round(size_t b) { return (b + 255) & ~size_t(255); }
     template <class T>
     hipError_t get(T** p, size_t count) {
-        void* q = nullptr;
-        const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 16);
-        if (e != hipSuccess) return e;
-        ptrs.push_back(q);
-        *p = static_cast<T*>(q);
+        const size_t b = round(count ? count * sizeof(T) : 16);
+        if (used + b > size) return hipErrorOutOfMemory;
+        *p = reinterpret_cast<T*>(base + used);
+        used += b;
         return hipSuccess;
     }
 };
@@ -835,11 +900,28 @@ inline uint32_t blocks(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 }  // namespace
 
+size_t scratch_bytes(uint32_t n) {
+    // Binary node slots: the level passes' nodes (at most 2n - 1) and each
+    // small segment's range (2 per primitive).
+    const size_t cap = 4 * size_t(n);
+    const size_t max_segs = n / kSmall + 2;
+    const size_t max_chunks = n / kChunk + max_segs + 2;
+    return DevMem::round(sizeof(float4) * n) + DevMem::round(4 * size_t(n)) + DevMem::round(sizeof(BNode) * cap) +
+           3 * DevMem::round(4 * cap) + DevMem::round(sizeof(int4) * cap) + DevMem::round(16) +
+           DevMem::round(sizeof(Seg) * max_segs) + DevMem::round(sizeof(SegAcc) * max_segs) +
+           DevMem::round(sizeof(uint2) * max_chunks) + DevMem::round(8 * max_chunks) +
+           DevMem::round(sizeof(SmallSeg) * size_t(n));
+}
+
 hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_nodes, uint32_t* n_nodes,
-                 hipStream_t st) {
-    if (n == 0) return hipErrorInvalidValue;
-    const uint32_t cap = 2 * n;   // binary nodes: at most 2n - 1
+                 void* d_scratch, size_t scratch_size, hipStream_t st) {
+    if (n == 0 || !d_scratch || scratch_size < scratch_bytes(n)) return hipErrorInvalidValue;
+    const uint32_t cap = 4 * n;   // binary node slots, as scratch_bytes counts them
+    const uint32_t max_segs = n / kSmall + 2;
+    const uint32_t max_chunks = n / kChunk + max_segs + 2;
     DevMem mem;
+    mem.base = static_cast<char*>(d_scratch);
+    mem.size = scratch_size;
     float4* centers;
     uint32_t *pos, *depth_of, *counters, *wsize, *cnt;
     BNode* nodes;
@@ -853,8 +935,6 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
     BVH_TRY(mem.get(&wsize, cap));
     BVH_TRY(mem.get(&index, cap));
     BVH_TRY(mem.get(&counters, 4));
-    const uint32_t max_segs = n / kSmall + 2;
-    const uint32_t max_chunks = n / kChunk + max_segs + 2;
     Seg* segs;
     SegAcc* acc;
     uint2* chunks;
@@ -862,11 +942,14 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
     BVH_TRY(mem.get(&acc, max_segs));
     BVH_TRY(mem.get(&chunks, max_chunks));
     BVH_TRY(mem.get(&cnt, 2 * max_chunks));
+    SmallSeg* d_small;   // small segments are disjoint and non-empty: at most n
+    BVH_TRY(mem.get(&d_small, n));
 
-    k_setup<<<blocks(n), kBlock, 0, st>>>(d_boxes, n, centers, d_order, counters);
+    k_setup<<<blocks(n), kBlock, 0, st>>>(d_boxes, n, centers, d_order, counters, depth_of, cap);
     BVH_TRY(hipGetLastError());
 
-    std::vector<uint4> small;
+    std::vector<SmallSeg> small;
+    uint32_t level_nodes = 1;   // the root, then two per split of a level pass
     std::vector<Seg> cur;
     auto add = [&](std::vector<Seg>& next, uint32_t first, uint32_t count, uint32_t depth, uint32_t node) {
         if (count > kSmall) {
@@ -877,7 +960,12 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
             s.node = node;
             next.push_back(s);
         } else {
-            small.push_back(make_uint4(first, count, depth, node));
+            SmallSeg s{};
+            s.first = first;
+            s.count = count;
+            s.depth = depth;
+            s.node = node;
+            small.push_back(s);
         }
     };
     add(cur, 0, n, 0, 0);
@@ -908,16 +996,22 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
         std::vector<Seg> next;
         for (const Seg& s : cur) {
             if (s.kind == 0) continue;
+            level_nodes += 2;
             add(next, s.first, s.mid - s.first, s.depth + 1, s.left);
             add(next, s.mid, s.first + s.count - s.mid, s.depth + 1, s.left + 1);
         }
         cur.swap(next);
     }
 
+    uint32_t total = level_nodes;
+    for (SmallSeg& s : small) {
+        s.base = total;
+        total += 2 * s.count - 2;
+    }
+    if (total > cap) return hipErrorInvalidValue;
     if (!small.empty()) {
-        uint4* d_small;
-        BVH_TRY(mem.get(&d_small, small.size()));
-        BVH_TRY(hipMemcpyAsync(d_small, small.data(), small.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+        if (small.size() > n) return hipErrorInvalidValue;
+        BVH_TRY(hipMemcpyAsync(d_small, small.data(), small.size() * sizeof(SmallSeg), hipMemcpyHostToDevice, st));
         k_small<<<static_cast<uint32_t>(small.size()), 64, 0, st>>>(d_boxes, centers, d_order, d_small, nodes, depth_of,
                                                                     counters);
         BVH_TRY(hipGetLastError());
@@ -925,10 +1019,10 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
     uint32_t hc[2];
     BVH_TRY(hipMemcpyAsync(hc, counters, 8, hipMemcpyDeviceToHost, st));
     BVH_TRY(hipStreamSynchronize(st));
-    const uint32_t total = hc[0], deepest = hc[1];
-    if (total > cap || deepest >= kMaxDepth) return hipErrorInvalidValue;
+    const uint32_t deepest = hc[1];
+    if (hc[0] != level_nodes || deepest >= kMaxDepth) return hipErrorInvalidValue;
 
-    k_clist<<<blocks(total), kBlock, 0, st>>>(nodes, total, clist);
+    k_clist<<<blocks(total), kBlock, 0, st>>>(nodes, depth_of, total, clist);
     k_index_init<<<blocks(total), kBlock, 0, st>>>(total, index);
     for (int d = static_cast<int>(deepest); d >= 0; --d)
         k_wsize<<<blocks(total), kBlock, 0, st>>>(nodes, depth_of, clist, total, static_cast<uint32_t>(d), wsize);
