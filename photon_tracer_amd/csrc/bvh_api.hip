@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -39,6 +40,12 @@ pt::bvh_tree device_bvh(const std::vector<pt::box3>& boxes) {
     const size_t n = boxes.size();
     if (n == 0) return out;
     if (n > kMaxBoxes) throw pt::status_error(PT_ERR_LIMIT, "BVH build: more than 2^30 primitives");
+    // With a NaN, std::min's result depends on the order it is taken in, so
+    // the parallel build could not promise the host's tree: refused.
+    for (const pt::box3& b : boxes)
+        for (int a = 0; a < 3; ++a)
+            if (std::isnan(b.lo[a]) || std::isnan(b.hi[a]))
+                throw pt::status_error(PT_ERR_INVALID, "device BVH build: a box has a NaN coordinate");
     Pool& P = pool();
     std::lock_guard<std::mutex> lock(P.m);
     P.boxes.ensure(24 * n);

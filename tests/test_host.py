@@ -138,6 +138,10 @@ def test_bvh_build_host_path():
     assert lib.pt_bvh_build(None, 3, 0, ctypes.byref(p1), ctypes.byref(n_nodes), ctypes.byref(p2), None) != 0
     w = pt.generate(0, 0, 0.25)
     assert lib.pt_pack_ex(w.ctypes.data, w.size, 2, ctypes.byref(p1), ctypes.byref(n_nodes)) != 0
+    bad = boxes.copy()
+    bad[7, 4] = np.nan   # refused before any device call
+    with pytest.raises(pt.PTError, match="NaN"):
+        pt.bvh_build(bad, device=True)
 
 
 def test_cycle_is_refused():
