@@ -102,8 +102,9 @@ pt_status pt_pack_ex(const uint32_t* ptsd, size_t n_words, int flags, uint32_t**
  * partition, one wave per small subtree, parallel collapse) with the same
  * result word for word; 0 builds on host threads. ms: wall-clock of the
  * build, the device's transfers and allocations included, the copies into
- * the returned arrays not (may be NULL). At most 2^30 boxes; the device
- * build refuses boxes with a NaN coordinate (PT_ERR_INVALID). */
+ * the returned arrays not (may be NULL). At most 2^30 boxes (2^28 for the
+ * device build, PT_ERR_LIMIT); the device build refuses boxes with a NaN
+ * coordinate (PT_ERR_INVALID). */
 enum { PT_BVH_DEVICE = 1 };
 pt_status pt_bvh_build(const float* boxes, size_t n, int flags, uint32_t** nodes, size_t* n_nodes, uint32_t** order,
                        double* ms);

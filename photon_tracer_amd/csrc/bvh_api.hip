@@ -39,7 +39,8 @@ pt::bvh_tree device_bvh(const std::vector<pt::box3>& boxes) {
     pt::bvh_tree out;
     const size_t n = boxes.size();
     if (n == 0) return out;
-    if (n > kMaxBoxes) throw pt::status_error(PT_ERR_LIMIT, "BVH build: more than 2^30 primitives");
+    if (n > ptd::bvhg::kMaxDeviceBoxes)
+        throw pt::status_error(PT_ERR_LIMIT, "device BVH build: more than 2^28 primitives");
     // With a NaN, std::min's result depends on the order it is taken in, so
     // the parallel build could not promise the host's tree: refused.
     for (const pt::box3& b : boxes)

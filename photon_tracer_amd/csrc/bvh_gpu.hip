@@ -915,7 +915,7 @@ size_t scratch_bytes(uint32_t n) {
 
 hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_nodes, uint32_t* n_nodes,
                  void* d_scratch, size_t scratch_size, hipStream_t st) {
-    if (n == 0 || !d_scratch || scratch_size < scratch_bytes(n)) return hipErrorInvalidValue;
+    if (n == 0 || n > kMaxDeviceBoxes || !d_scratch || scratch_size < scratch_bytes(n)) return hipErrorInvalidValue;
     const uint32_t cap = 4 * n;   // binary node slots, as scratch_bytes counts them
     const uint32_t max_segs = n / kSmall + 2;
     const uint32_t max_chunks = n / kChunk + max_segs + 2;

@@ -10,13 +10,17 @@
 namespace ptd {
 namespace bvhg {
 
+// Most boxes one device build takes: node slots (4 per box) stay below
+// 2^32 and the scratch (about 270 bytes a box) within one GPU's memory.
+constexpr uint32_t kMaxDeviceBoxes = 1u << 28;
+
 // Device scratch a build over n boxes needs (bytes).
 size_t scratch_bytes(uint32_t n);
 
 // Builds the tree over n boxes. d_boxes: n boxes on the device, 6 floats
 // each (min xyz, max xyz: the aabb layout, aabb.hpp:9). Outputs on the
 // device: d_order (n words) and d_nodes (128-byte 4-wide nodes, room for
-// max(n, 1)); *n_nodes is the number written. n >= 1. d_scratch: at least
+// max(n, 1)); *n_nodes is the number written. 1 <= n <= kMaxDeviceBoxes. d_scratch: at least
 // scratch_bytes(n) bytes of device memory, 256-byte aligned. Synchronises
 // `stream` before returning.
 hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_nodes, uint32_t* n_nodes,
