@@ -81,6 +81,19 @@ typedef struct {
 } pt_job_desc;
 pt_status pt_generate_job(uint64_t seed, double scale, pt_job_desc* desc, uint32_t** ptsd, size_t* n_words);
 
+/* load_obj (obj.hpp:114-218) as the browser demo calls it, with no file
+ * reader: the faces of OBJ text (polygons split into fans; v, vt, vn and
+ * negative indices), turned turn_deg about the vertical axis, scaled to
+ * `height` and set down with the bottom of their bounding box centred on
+ * `base`, appended to a PTSD scene's triangles with material index
+ * `material`. Normals from the file where every corner of a face has one,
+ * otherwise area-weighted vertex normals; degenerate faces dropped. mtllib and
+ * usemtl are not read (no reader), and PTGP carries no textures
+ * (gpu_pack.hpp:96). Returns a new PTSD buffer (free with pt_free). */
+pt_status pt_ptsd_add_obj(const uint32_t* ptsd, size_t n_words, const char* obj_text, size_t text_len,
+                          const float base[3], double height, uint32_t material, double turn_deg, uint32_t** out,
+                          size_t* out_words);
+
 /* Builds every BVH of a PTSD scene (binned SAH, 16 bins, max leaf 8, 4-wide
  * collapse; bvh.hpp:39-52,116-247; geometry.hpp:74-104) and packs the scene
  * as PTGP v1 (pack_for_gpu + serialize, gpu_pack.hpp:84-203). */

@@ -141,6 +141,8 @@ def library() -> ctypes.CDLL:
     lib.pt_generate_job.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(_JobDesc),
                                     ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
     lib.pt_pack.argtypes = [vp, sz, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
+    lib.pt_ptsd_add_obj.argtypes = [vp, sz, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_float), ctypes.c_double,
+                                    ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
     lib.pt_pack_ex.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
     lib.pt_bvh_build.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz), ctypes.POINTER(_u32p),
                                  ctypes.POINTER(ctypes.c_double)]
@@ -181,7 +183,7 @@ def library() -> ctypes.CDLL:
     for name in ("pt_batch_merge", "pt_batch_create", "pt_batch_destroy", "pt_render_batch", "pt_batch_autotune"):
         getattr(lib, name).restype = ctypes.c_int
     for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate",
-                 "pt_generate_job", "pt_pack", "pt_pack_ex", "pt_bvh_build", "pt_ptgp_depth", "pt_scene_create",
+                 "pt_generate_job", "pt_pack", "pt_pack_ex", "pt_bvh_build", "pt_ptsd_add_obj", "pt_ptgp_depth", "pt_scene_create",
                  "pt_scene_destroy", "pt_trace",
                  "pt_trace_device", "pt_render", "pt_guide_create", "pt_guide_destroy", "pt_scene_set_guide",
                  "pt_guide_get_weights", "pt_guide_set_weights", "pt_guide_eval", "pt_guide_pdf", "pt_guide_sample",
@@ -257,6 +259,18 @@ def pack(ptsd, device_bvh: bool = False) -> np.ndarray:
         _check(library().pt_pack_ex(w.ctypes.data, w.size, 1, ctypes.byref(p), ctypes.byref(n)))
     else:
         _check(library().pt_pack(w.ctypes.data, w.size, ctypes.byref(p), ctypes.byref(n)))
+    return _take_words(p, n)
+
+
+def add_obj(ptsd, obj_text, base, height: float, material: int, turn_deg: float = 0.0) -> np.ndarray:
+    """load_obj without a file reader: the OBJ text's triangles appended to a
+    PTSD scene with the given material index; returns the new PTSD words."""
+    w = _words(ptsd)
+    data = obj_text.encode() if isinstance(obj_text, str) else bytes(obj_text)
+    b = (ctypes.c_float * 3)(*[float(x) for x in base])
+    p, n = _u32p(), ctypes.c_size_t(0)
+    _check(library().pt_ptsd_add_obj(w.ctypes.data, w.size, data, len(data), b, float(height), int(material),
+                                     float(turn_deg), ctypes.byref(p), ctypes.byref(n)))
     return _take_words(p, n)
 
 

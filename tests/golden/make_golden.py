@@ -9,6 +9,9 @@
   reference.json.gz   the reference's statistical render fixture
                       (tests/reference.json: per 20x20 block mean and sigma at
                       960x540, 200 spp, 16 seeds), copied unchanged.
+  teapot.obj.gz       the Utah teapot the reference's mesh scene loads
+                      (models/teapot.obj, renderer.hpp:221-231), copied
+                      unchanged: the OBJ loader's input.
 
 Run: python tests/golden/make_golden.py
 """
@@ -42,6 +45,8 @@ def main():
         print(name, ptsd.size, ptgp.size)
     np.savez_compressed(os.path.join(HERE, "ref_scenes.npz"), **arrays)
     with open(f"{REF}/tests/reference.json", "rb") as src, gzip.open(os.path.join(HERE, "reference.json.gz"), "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    with open(f"{REF}/models/teapot.obj", "rb") as src, gzip.open(os.path.join(HERE, "teapot.obj.gz"), "wb") as dst:
         shutil.copyfileobj(src, dst)
 
 

@@ -168,3 +168,67 @@ def test_pfm_round_trip(tmp_path):
     # bottom row first (image_write.hpp:24-28)
     first = np.frombuffer(raw[len(b"PF\n7 5\n-1.0\n"):][: 7 * 12], "<f4").reshape(7, 3)
     assert np.array_equal(first, img[-1])
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def ptsd_triangles(w):
+    """The flat triangle records of a PTSD buffer: 28 words each, after the
+    32-word header, 12-word materials and 5-word spheres and large spheres."""
+    start = 32 + 12 * int(w[10]) + 5 * (int(w[11]) + int(w[12]))
+    return w[start:start + 28 * int(w[13])].reshape(-1, 28)
+
+
+def test_obj_loader_matches_reference_load_obj():
+    # The teapot scene's fixture holds the reference's own load_obj output
+    # (renderer.hpp:230: the Utah teapot, base (0, -0.5, -1.2), height 0.75,
+    # copper, no file reader, no turn). Our loader's triangles of the same
+    # OBJ text, appended to that scene, must repeat them word for word.
+    import gzip
+    ptsd, _ = rc.load_scene("teapot")
+    text = gzip.open(os.path.join(GOLDEN, "teapot.obj.gz")).read()
+    ref = ptsd_triangles(ptsd)
+    copper = int(ref[0, 24])
+    assert (ref[:, 24] == copper).all() and len(ref) > 1000
+    out = pt.add_obj(ptsd, text, (0.0, -0.5, -1.2), 0.75, copper)
+    tris = ptsd_triangles(out)
+    assert len(tris) == 2 * len(ref)
+    assert np.array_equal(tris[:len(ref)], ref)
+    assert np.array_equal(tris[len(ref):], ref)
+    assert pt.pack(out).size > pt.pack(ptsd).size
+
+
+def test_obj_loader_faces_normals_and_errors():
+    # A quad (a fan of two triangles) with file normals on one face only, a
+    # negative index, texture coordinates, and a degenerate face that is
+    # dropped; turned 90 degrees and scaled to height 2.
+    ptsd = pt.generate(0, 0, 0.05)
+    n0 = int(ptsd[13])
+    text = b"""v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+v 2 2 2
+vt 0.25 0.5
+vn 0 0 1
+f 1/1/1 2/1/1 3/1/1 4/1/1
+f -1 -1 -1
+f 1 2 5
+"""
+    out = pt.add_obj(ptsd, text, (0.0, 0.0, 0.0), 2.0, 0, turn_deg=90)
+    tris = ptsd_triangles(out)[n0:]
+    assert len(tris) == 3                     # the degenerate face is dropped
+    f = tris.view(np.float32)
+    # turned 90 degrees about y ((x, z) -> (z, -x)), scaled to height 2 and set down on the origin: y spans [0, 2]
+    assert np.isclose(f[:, [1, 4, 7]].min(), 0) and np.isclose(f[:, [1, 4, 7]].max(), 2)
+    # the quad's corners carry the file normal (0, 0, 1) turned to (1, 0, 0) and the texture coordinate
+    assert np.allclose(f[0, 9:18].reshape(3, 3), [[1, 0, 0]] * 3, atol=1e-6)
+    assert np.allclose(f[0, 18:24], [0.25, 0.5] * 3)
+    # the third face has no file normals: unit area-weighted vertex normals
+    assert np.allclose(np.linalg.norm(f[2, 9:18].reshape(3, 3), axis=1), 1, atol=1e-6)
+    with pytest.raises(pt.PTError, match="out of range"):
+        pt.add_obj(ptsd, b"v 0 0 0\nf 1 2 3\n", (0, 0, 0), 1.0, 0)
+    with pytest.raises(pt.PTError):
+        pt.add_obj(ptsd, text, (0, 0, 0), 1.0, 10_000)        # no such material
+    assert np.array_equal(pt.add_obj(ptsd, b"# nothing\n", (0, 0, 0), 1.0, 0), ptsd)
