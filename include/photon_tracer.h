@@ -81,18 +81,21 @@ typedef struct {
 } pt_job_desc;
 pt_status pt_generate_job(uint64_t seed, double scale, pt_job_desc* desc, uint32_t** ptsd, size_t* n_words);
 
-/* load_obj (obj.hpp:114-218) as the browser demo calls it, with no file
- * reader: the faces of OBJ text (polygons split into fans; v, vt, vn and
- * negative indices), turned turn_deg about the vertical axis, scaled to
- * `height` and set down with the bottom of their bounding box centred on
- * `base`, appended to a PTSD scene's triangles with material index
- * `material`. Normals from the file where every corner of a face has one,
- * otherwise area-weighted vertex normals; degenerate faces dropped. mtllib and
- * usemtl are not read (no reader), and PTGP carries no textures
+/* load_obj (obj.hpp:114-218): the faces of OBJ text (polygons split into
+ * fans; v, vt, vn and negative indices), turned turn_deg about the vertical
+ * axis, scaled to `height` and set down with the bottom of their bounding box
+ * centred on `base`, appended to a PTSD scene's triangles. Normals from the
+ * file where every corner of a face has one, otherwise area-weighted vertex
+ * normals; degenerate faces dropped. In place of the reference's file reader,
+ * mtl_text (may be NULL) is the text of the MTL library that every mtllib
+ * line names: usemtl picks its materials, mapped as make_material maps them
+ * (obj.hpp:89-112) and appended to the scene's materials; faces before any
+ * usemtl, or naming a material the library lacks, take `material` (an index
+ * into the scene's materials). Texture maps are not carried: PTGP has none
  * (gpu_pack.hpp:96). Returns a new PTSD buffer (free with pt_free). */
 pt_status pt_ptsd_add_obj(const uint32_t* ptsd, size_t n_words, const char* obj_text, size_t text_len,
-                          const float base[3], double height, uint32_t material, double turn_deg, uint32_t** out,
-                          size_t* out_words);
+                          const char* mtl_text, size_t mtl_len, const float base[3], double height, uint32_t material,
+                          double turn_deg, uint32_t** out, size_t* out_words);
 
 /* Builds every BVH of a PTSD scene (binned SAH, 16 bins, max leaf 8, 4-wide
  * collapse; bvh.hpp:39-52,116-247; geometry.hpp:74-104) and packs the scene

@@ -141,8 +141,9 @@ def library() -> ctypes.CDLL:
     lib.pt_generate_job.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(_JobDesc),
                                     ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
     lib.pt_pack.argtypes = [vp, sz, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
-    lib.pt_ptsd_add_obj.argtypes = [vp, sz, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_float), ctypes.c_double,
-                                    ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
+    lib.pt_ptsd_add_obj.argtypes = [vp, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_float),
+                                    ctypes.c_double, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(_u32p),
+                                    ctypes.POINTER(sz)]
     lib.pt_pack_ex.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
     lib.pt_bvh_build.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz), ctypes.POINTER(_u32p),
                                  ctypes.POINTER(ctypes.c_double)]
@@ -262,15 +263,17 @@ def pack(ptsd, device_bvh: bool = False) -> np.ndarray:
     return _take_words(p, n)
 
 
-def add_obj(ptsd, obj_text, base, height: float, material: int, turn_deg: float = 0.0) -> np.ndarray:
-    """load_obj without a file reader: the OBJ text's triangles appended to a
-    PTSD scene with the given material index; returns the new PTSD words."""
+def add_obj(ptsd, obj_text, base, height: float, material: int, turn_deg: float = 0.0, mtl_text=None) -> np.ndarray:
+    """load_obj: the OBJ text's triangles appended to a PTSD scene, with
+    materials from mtl_text (the library every mtllib names) or the given
+    material index; returns the new PTSD words."""
     w = _words(ptsd)
     data = obj_text.encode() if isinstance(obj_text, str) else bytes(obj_text)
+    mtl = None if mtl_text is None else (mtl_text.encode() if isinstance(mtl_text, str) else bytes(mtl_text))
     b = (ctypes.c_float * 3)(*[float(x) for x in base])
     p, n = _u32p(), ctypes.c_size_t(0)
-    _check(library().pt_ptsd_add_obj(w.ctypes.data, w.size, data, len(data), b, float(height), int(material),
-                                     float(turn_deg), ctypes.byref(p), ctypes.byref(n)))
+    _check(library().pt_ptsd_add_obj(w.ctypes.data, w.size, data, len(data), mtl, 0 if mtl is None else len(mtl), b,
+                                     float(height), int(material), float(turn_deg), ctypes.byref(p), ctypes.byref(n)))
     return _take_words(p, n)
 
 

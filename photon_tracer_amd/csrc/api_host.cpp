@@ -59,15 +59,16 @@ pt_status pt_generate_job(uint64_t seed, double scale, pt_job_desc* desc, uint32
 }
 
 pt_status pt_ptsd_add_obj(const uint32_t* ptsd, size_t n_words, const char* obj_text, size_t text_len,
-                          const float base[3], double height, uint32_t material, double turn_deg, uint32_t** out,
-                          size_t* out_words) {
-    if (!ptsd || !out || !out_words || !base || (text_len && !obj_text))
+                          const char* mtl_text, size_t mtl_len, const float base[3], double height, uint32_t material,
+                          double turn_deg, uint32_t** out, size_t* out_words) {
+    if (!ptsd || !out || !out_words || !base || (text_len && !obj_text) || (mtl_len && !mtl_text))
         return pt::fail(PT_ERR_INVALID, "pt_ptsd_add_obj: null pointer");
     if (!(height > 0)) return pt::fail(PT_ERR_INVALID, "pt_ptsd_add_obj: height must be > 0");
     PT_GUARD({
         pt::scene_desc s = pt::ptsd_read(ptsd, n_words);
+        const std::string mtl(mtl_text ? mtl_text : "", mtl_len);
         pt::add_obj(s, std::string(obj_text ? obj_text : "", text_len), pt::v3(base[0], base[1], base[2]), height,
-                    material, turn_deg);
+                    material, turn_deg, mtl_text ? &mtl : nullptr);
         *out = pt::copy_out(pt::ptsd_write(s), out_words);
     });
 }

@@ -175,9 +175,11 @@ box3 triangle_box(const triangle_rec& t);
 using bvh_builder = std::function<bvh_tree(const std::vector<box3>&)>;
 std::vector<uint32_t> pack_ptgp(const scene_desc& s, const bvh_builder* builder = nullptr);
 
-// load_obj (obj.hpp:114-218) without a file reader: the OBJ text's faces
-// appended to s.triangles with material `material` (obj.cpp).
-void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uint32_t material, double turn_deg);
+// load_obj (obj.hpp:114-218): the OBJ text's faces appended to s.triangles,
+// with materials from the MTL text (standing in for every library the OBJ
+// names; appended to s.materials) or `material` (obj.cpp).
+void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uint32_t material, double turn_deg,
+             const std::string* mtl_text = nullptr);
 
 // A malloc'd copy of v for the C ABI (free with pt_free); *n = v.size().
 uint32_t* copy_out(const std::vector<uint32_t>& v, size_t* n);

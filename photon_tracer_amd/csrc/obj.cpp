@@ -1,8 +1,10 @@
 // OBJ text to triangles (load_obj, obj.hpp:114-218), for scenes that the
-// packer and the device path take as PTSD. Called without a file reader, as
-// the browser demo calls it: mtllib is not read, so every face takes the
-// given material, and there are no texture images (PTGP carries none,
-// gpu_pack.hpp:96).
+// packer and the device path take as PTSD. Instead of a file reader, the
+// caller passes the text of one MTL library, which stands in for every
+// library the OBJ names (none: every face takes the given material, as the
+// browser demo loads). MTL materials map onto the four kinds as
+// make_material does (obj.hpp:89-112); texture maps are not carried (PTGP
+// has none, gpu_pack.hpp:96), so a textured matte keeps its Kd.
 //
 // The arithmetic follows the reference's types step by step, so the
 // triangles come out bit for bit: coordinates read as double and stored as
@@ -11,6 +13,7 @@
 // their per-vertex sums in float, in face order; float normalisation.
 #include <array>
 #include <cmath>
+#include <map>
 #include <sstream>
 
 #include "host.hpp"
@@ -44,13 +47,94 @@ int corner_index(const std::string& s, size_t count) {
     return static_cast<int>(k);
 }
 
+struct mtl_rec {
+    double kd[3] = {0.8, 0.8, 0.8}, ks[3] = {0, 0, 0}, ke[3] = {0, 0, 0};
+    double ns = 0, ni = 1.5, d = 1;
+    int illum = 2;
+};
+
+// read_mtl (obj.hpp:49-87), less map_Kd.
+void read_mtl(const std::string& text, std::map<std::string, mtl_rec>& out) {
+    std::istringstream in(text);
+    std::string line;
+    mtl_rec* cur = nullptr;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string tag;
+        ls >> tag;
+        if (tag == "newmtl") {
+            std::string name;
+            ls >> name;
+            cur = &out[name];
+            *cur = mtl_rec();
+        } else if (!cur) {
+            continue;
+        } else if (tag == "Kd" || tag == "Ks" || tag == "Ke") {
+            double r, g, b;
+            if (!(ls >> r)) continue;
+            if (!(ls >> g >> b)) g = b = r;   // one value means gray
+            double* c = tag == "Kd" ? cur->kd : tag == "Ks" ? cur->ks : cur->ke;
+            c[0] = r;
+            c[1] = g;
+            c[2] = b;
+        } else if (tag == "Ns") {
+            ls >> cur->ns;
+        } else if (tag == "Ni") {
+            ls >> cur->ni;
+        } else if (tag == "d") {
+            ls >> cur->d;
+        } else if (tag == "Tr") {
+            double tr;
+            if (ls >> tr) cur->d = 1 - tr;
+        } else if (tag == "illum") {
+            ls >> cur->illum;
+        }
+    }
+}
+
+// make_material (obj.hpp:89-112): emission makes a light; dissolve below 1
+// or a glass model (4, 6, 7, 9) glass; a reflection model (3, 5) metal with
+// a fuzz from the shininess; anything else matte. Appended to s.materials.
+uint32_t make_material(scene_desc& s, const mtl_rec& m) {
+    auto nonzero = [](const double* c) { return c[0] > 0 || c[1] > 0 || c[2] > 0; };
+    auto color = [](const double* c) {
+        return v3(static_cast<float>(c[0]), static_cast<float>(c[1]), static_cast<float>(c[2]));
+    };
+    material_rec r;
+    if (nonzero(m.ke)) {
+        r.type = 3;
+        r.emission = color(m.ke);
+    } else if (m.d < 1 || m.illum == 4 || m.illum == 6 || m.illum == 7 || m.illum == 9) {
+        r.type = 2;
+        r.ior = static_cast<float>(m.ni > 1 ? m.ni : 1.5);
+    } else if (m.illum == 3 || m.illum == 5) {
+        r.type = 1;
+        r.albedo = color(nonzero(m.ks) ? m.ks : m.kd);
+        const float f = static_cast<float>(std::min(1.0, std::sqrt(2 / (m.ns + 2))));
+        r.fuzz = f < 1 ? f : 1;   // metal's constructor clamp (material.hpp:83)
+    } else {
+        r.type = 0;
+        r.albedo = color(m.kd);
+    }
+    s.materials.push_back(r);
+    return static_cast<uint32_t>(s.materials.size() - 1);
+}
+
 }  // namespace
 
-void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uint32_t material, double turn_deg) {
+void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uint32_t material, double turn_deg,
+             const std::string* mtl_text) {
     if (material >= s.materials.size()) throw std::runtime_error("OBJ material index out of range");
+    std::map<std::string, mtl_rec> library;
+    std::map<std::string, uint32_t> made;
+    uint32_t current = material;
     std::vector<v3> verts, file_normals;
     std::vector<std::array<float, 2>> uvs;
-    std::vector<std::array<corner, 3>> faces;
+    struct face {
+        std::array<corner, 3> c;
+        uint32_t material;
+    };
+    std::vector<face> faces;
 
     std::istringstream in(text);
     std::string line;
@@ -67,6 +151,20 @@ void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uin
             double u = 0, v = 0;
             ls >> u >> v;
             uvs.push_back({static_cast<float>(u), static_cast<float>(v)});
+        } else if (tag == "mtllib" && mtl_text) {
+            std::string name;
+            while (ls >> name) read_mtl(*mtl_text, library);
+        } else if (tag == "usemtl") {
+            std::string name;
+            ls >> name;
+            const auto found = library.find(name);
+            if (found == library.end()) {
+                current = material;
+            } else {
+                auto m = made.find(name);
+                if (m == made.end()) m = made.emplace(name, make_material(s, found->second)).first;
+                current = m->second;
+            }
         } else if (tag == "f") {
             std::vector<corner> poly;
             std::string tok;
@@ -79,7 +177,7 @@ void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uin
                 c.vn = s2 == std::string::npos ? -1 : corner_index(tok.substr(s2 + 1), file_normals.size());
                 poly.push_back(c);
             }
-            for (size_t k = 2; k < poly.size(); ++k) faces.push_back({poly[0], poly[k - 1], poly[k]});
+            for (size_t k = 2; k < poly.size(); ++k) faces.push_back({{poly[0], poly[k - 1], poly[k]}, current});
         }
     }
     if (verts.empty() || faces.empty()) return;
@@ -107,13 +205,15 @@ void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uin
     // Area-weighted vertex normals: the cross product's length is twice the
     // face's area.
     std::vector<v3> smooth(verts.size());
-    for (const auto& f : faces) {
+    for (const auto& face : faces) {
+        const auto& f = face.c;
         const v3 n = cross3(sub3(verts[f[1].v], verts[f[0].v]), sub3(verts[f[2].v], verts[f[0].v]));
         for (const corner& c : f) smooth[c.v] = add3(smooth[c.v], n);
     }
 
     s.triangles.reserve(s.triangles.size() + faces.size());
-    for (const auto& f : faces) {
+    for (const auto& face : faces) {
+        const auto& f = face.c;
         const v3 &a = verts[f[0].v], &b = verts[f[1].v], &c = verts[f[2].v];
         const v3 fn = cross3(sub3(b, a), sub3(c, a));
         if (dot3(fn, fn) < 1e-24) continue;   // degenerate
@@ -129,7 +229,7 @@ void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uin
                 t.uv[2 * k + 1] = uvs[f[k].vt][1];
             }
         }
-        t.material = material;
+        t.material = face.material;
         s.triangles.push_back(t);
     }
 }

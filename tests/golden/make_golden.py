@@ -12,6 +12,16 @@
   teapot.obj.gz       the Utah teapot the reference's mesh scene loads
                       (models/teapot.obj, renderer.hpp:221-231), copied
                       unchanged: the OBJ loader's input.
+  spot_scene.npz,     the reference's mesh scene loading Keenan Crane's Spot
+  spot.obj.gz,        through its file reader (models/spot: the OBJ names
+  spot.mtl            spot.mtl, a textured matte), exported as PTSD, and the
+                      two input files copied unchanged: the MTL path's fixture.
+  mtl_kinds.npz       a small OBJ and MTL of our own (one quad per material:
+                      a light, glass by dissolve, by Tr and by illum 7, metals
+                      from illum 3 and 5 with and without Ks, a gray Kd from
+                      one value, an unknown name) loaded by the reference's
+                      mesh scene through its file reader, as PTSD, with the
+                      two texts: every branch of make_material.
 
 Run: python tests/golden/make_golden.py
 """
@@ -48,6 +58,60 @@ def main():
         shutil.copyfileobj(src, dst)
     with open(f"{REF}/models/teapot.obj", "rb") as src, gzip.open(os.path.join(HERE, "teapot.obj.gz"), "wb") as dst:
         shutil.copyfileobj(src, dst)
+    write_spot()
+    write_mtl_kinds()
+
+
+def write_spot():
+    ptsd, _ = ob.ref_scene("mesh", obj_path=f"{REF}/models/spot/spot.obj")
+    np.savez_compressed(os.path.join(HERE, "spot_scene.npz"), spot_ptsd=ptsd)
+    with open(f"{REF}/models/spot/spot.obj", "rb") as src, gzip.open(os.path.join(HERE, "spot.obj.gz"), "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    shutil.copyfile(f"{REF}/models/spot/spot.mtl", os.path.join(HERE, "spot.mtl"))
+    print("spot", ptsd.size)
+
+
+MTL_KINDS = """newmtl lamp
+Ke 4 3 2
+newmtl clear
+d 0.5
+Ni 1.33
+newmtl frosted
+Tr 0.25
+newmtl crystal
+illum 7
+Ni 0.9
+newmtl copper
+illum 3
+Ks 0.9 0.6 0.5
+Ns 40
+newmtl steel
+illum 5
+Kd 0.6 0.6 0.7
+newmtl chalk
+Kd 0.7
+newmtl rough
+illum 3
+Ns 0
+"""
+
+
+def write_mtl_kinds():
+    import tempfile
+    names = ["lamp", "clear", "frosted", "crystal", "copper", "steel", "chalk", "rough", "nosuch"]
+    lines = ["mtllib kinds.mtl"]
+    for k, name in enumerate(names):
+        x = 2.0 * k
+        lines += [f"v {x} 0 0", f"v {x + 1} 0 0", f"v {x + 1} 1 0.5", f"v {x} 1 0.25", f"usemtl {name}",
+                  f"f {4 * k + 1} {4 * k + 2} {4 * k + 3} {4 * k + 4}"]
+    obj = "\n".join(lines) + "\n"
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "kinds.obj"), "w").write(obj)
+        open(os.path.join(d, "kinds.mtl"), "w").write(MTL_KINDS)
+        ptsd, _ = ob.ref_scene("mesh", obj_path=os.path.join(d, "kinds.obj"))
+    np.savez_compressed(os.path.join(HERE, "mtl_kinds.npz"), ptsd=ptsd,
+                        obj=np.frombuffer(obj.encode(), np.uint8), mtl=np.frombuffer(MTL_KINDS.encode(), np.uint8))
+    print("mtl kinds", ptsd.size)
 
 
 if __name__ == "__main__":

@@ -232,3 +232,48 @@ f 1 2 5
     with pytest.raises(pt.PTError):
         pt.add_obj(ptsd, text, (0, 0, 0), 1.0, 10_000)        # no such material
     assert np.array_equal(pt.add_obj(ptsd, b"# nothing\n", (0, 0, 0), 1.0, 0), ptsd)
+
+
+def test_obj_loader_with_mtl_matches_reference_load_obj():
+    # The reference's mesh scene loading Spot through its file reader: the OBJ
+    # names spot.mtl (a matte with a texture map). Our loader, given the same
+    # MTL text, must add the same triangles and a material with the same
+    # words; the texture is not carried by PTSD/PTGP on either side.
+    import gzip
+    ref_ptsd = np.load(os.path.join(GOLDEN, "spot_scene.npz"))["spot_ptsd"]
+    text = gzip.open(os.path.join(GOLDEN, "spot.obj.gz")).read()
+    mtl = open(os.path.join(GOLDEN, "spot.mtl"), "rb").read()
+    ref = ptsd_triangles(ref_ptsd)
+    n_mat = int(ref_ptsd[10])
+    out = pt.add_obj(ref_ptsd, text, (0.0, -0.5, -1.2), 0.75, 0, mtl_text=mtl)
+    assert int(out[10]) == n_mat + 1                       # one material made, appended
+    tris = ptsd_triangles(out)[len(ref):]
+    assert len(tris) == len(ref) and (tris[:, 24] == n_mat).all()
+    assert np.array_equal(np.delete(tris, 24, axis=1), np.delete(ref, 24, axis=1))
+    mats = out[32:32 + 12 * (n_mat + 1)].reshape(-1, 12)
+    assert np.array_equal(mats[n_mat], mats[int(ref[0, 24])])
+    assert mats[n_mat, 3] == 0 and np.array_equal(mats[n_mat, :3].view(np.float32), [1, 1, 1])   # matte, Kd 1 1 1
+    # Without the MTL text every face takes the given material, as the browser demo loads.
+    plain = ptsd_triangles(pt.add_obj(ref_ptsd, text, (0.0, -0.5, -1.2), 0.75, 2))[len(ref):]
+    assert (plain[:, 24] == 2).all()
+
+
+def test_obj_loader_mtl_kinds_match_reference_make_material():
+    # Every branch of make_material (obj.hpp:89-112), from an OBJ and MTL of
+    # our own that the reference's mesh scene loaded through its file reader
+    # (tests/golden/make_golden.py): the same triangles and, material by
+    # material, the same words.
+    z = np.load(os.path.join(GOLDEN, "mtl_kinds.npz"))
+    ref_ptsd = z["ptsd"]
+    ref = ptsd_triangles(ref_ptsd)
+    n_mat = int(ref_ptsd[10])
+    fallback = int(ref[-1, 24])   # the unknown name's faces: the scene's copper, the reference's fallback
+    out = pt.add_obj(ref_ptsd, z["obj"].tobytes(), (0.0, -0.5, -1.2), 0.75, fallback, mtl_text=z["mtl"].tobytes())
+    tris = ptsd_triangles(out)[len(ref):]
+    assert len(tris) == len(ref) == 18
+    assert np.array_equal(np.delete(tris, 24, axis=1), np.delete(ref, 24, axis=1))
+    mats = out[32:32 + 12 * int(out[10])].reshape(-1, 12)
+    ref_mats = ref_ptsd[32:32 + 12 * n_mat].reshape(-1, 12)
+    for ours, theirs in zip(tris[:, 24], ref[:, 24]):
+        assert np.array_equal(mats[ours], ref_mats[theirs])
+    assert len(set(tris[:16, 24])) == 8 and set(mats[tris[:16:2, 24], 3]) == {0, 1, 2, 3}
