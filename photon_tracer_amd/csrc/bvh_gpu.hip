@@ -36,10 +36,11 @@ constexpr uint32_t kMaxDepth = 64;
 constexpr float kTraversalCost = 1, kIntersectionCost = 1;
 
 // Longest segment one wave finishes in LDS. The wave's work is a chain of
-// dependent LDS steps per node, so what pays is waves per CU: 256 keeps the
-// LDS copy near 14 KB (11 waves per CU); 1024 (46 KB, 3 waves) took 19.6 ms
-// of a 22.6 ms build of 1M boxes.
-constexpr uint32_t kSmall = 256;
+// dependent LDS steps per node, so what pays is waves per CU: 128 keeps the
+// wave's LDS near 8 KB. For 1M boxes the subtree kernel took 19.6 ms at 1024
+// (46 KB, 3 waves per CU), 3.5 ms at 256 (14 KB) and 3.0 ms at 128, where one
+// more level pass costs 0.13 ms.
+constexpr uint32_t kSmall = 128;
 constexpr uint32_t kChunk = 4096;   // primitives per workgroup in the level passes
 constexpr int kBlock = 256;
 constexpr int kBinWords = 3 * kBins * 7;   // per axis and bin: min xyz, max xyz (keys), count
