@@ -175,6 +175,23 @@ pt_status pt_trace(pt_scene* scene, const pt_ray* rays, size_t n, int any_hit, p
 pt_status pt_trace_device(pt_scene* scene, const void* d_rays, size_t n, int any_hit, void* d_hits,
                           float* elapsed_ms, uint64_t* visits);
 
+/* pt_trace_device without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The traversal's work counter and stack spill area live in a 64 MiB array
+ * the library holds on each device it runs on until the process ends; the
+ * caller passes the rays and hits (device addresses, as pt_trace_device).
+ * stream: a hipStream_t to enqueue on, returning without waiting (the call may
+ * be captured into a graph), or NULL for the scene's stream, waited for before
+ * returning. The scene's device must be current. Every call on one device
+ * shares that device's array: calls that overlap on one device race (enqueue
+ * them on one stream). Scenes whose trees need more stack than the array
+ * holds for one traversal block per compute unit are refused (PT_ERR_LIMIT);
+ * deep trees run on a smaller grid than pt_trace_device's. Hits are bit for
+ * bit those of pt_trace_device. */
+pt_status pt_trace_device_noalloc(pt_scene* scene, const void* d_rays, size_t n, int any_hit, void* d_hits,
+                                  void* stream);
+
 /* ---------------------------------------------------------------------------
  * Rendering: render_row_linear (renderer.hpp:411-416) for a whole frame, or a
  * band of rows, as a wavefront path loop on the GPU.

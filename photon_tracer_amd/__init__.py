@@ -153,6 +153,7 @@ def library() -> ctypes.CDLL:
     lib.pt_trace.argtypes = [vp, vp, sz, ctypes.c_int, vp]
     lib.pt_trace_device.argtypes = [vp, vp, sz, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_float),
                                     ctypes.POINTER(ctypes.c_uint64)]
+    lib.pt_trace_device_noalloc.argtypes = [vp, vp, sz, ctypes.c_int, vp, vp]
     lib.pt_render.argtypes = [vp, ctypes.POINTER(_RenderParams), vp, vp, ctypes.POINTER(_RenderStats)]
     lib.pt_denoise.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int,
                                ctypes.POINTER(ctypes.c_double)]
@@ -186,7 +187,7 @@ def library() -> ctypes.CDLL:
     for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate",
                  "pt_generate_job", "pt_pack", "pt_pack_ex", "pt_bvh_build", "pt_ptsd_add_obj", "pt_ptgp_depth", "pt_scene_create",
                  "pt_scene_destroy", "pt_trace",
-                 "pt_trace_device", "pt_render", "pt_guide_create", "pt_guide_destroy", "pt_scene_set_guide",
+                 "pt_trace_device", "pt_trace_device_noalloc", "pt_render", "pt_guide_create", "pt_guide_destroy", "pt_scene_set_guide",
                  "pt_guide_get_weights", "pt_guide_set_weights", "pt_guide_eval", "pt_guide_pdf", "pt_guide_sample",
                  "pt_guide_gradient", "pt_guide_step", "pt_guide_stats"):
         getattr(lib, name).restype = ctypes.c_int
