@@ -11,7 +11,9 @@
 //                 leaf, primitive and instance visits per query.
 //   or_trace_f64  the same closest-hit query in double precision, each
 //                 primitive tested in FP64 and the tree used only to cull
-//                 (boxes padded), as an independent check of t and primitive.
+//                 (boxes padded), as an independent check of t and primitive;
+//                 the ray enters an instance's object space in double, or
+//                 (float_ray) in float as the reference moves it.
 //   or_sample_2d  the shuffled Owen-scrambled Sobol sampler
 //                 (sampler.hpp:35-127), with the reference's byte tables.
 //   or_render     ray_color, direct_light, the materials and
@@ -399,8 +401,11 @@ void traverse64(const scene& s, uint32_t root, dv o, dv d, double t_min, double&
     }
 }
 
+// float_ray: move the ray into an instance's object space in float, as the
+// reference does (geometry.hpp:213-217), and only then intersect in double;
+// otherwise the object-space ray is computed in double too.
 void trace64(const scene& s, V of, V df, double t_min, double t_max, double& t_out, uint32_t& kind, uint32_t& prim,
-             uint32_t& inst) {
+             uint32_t& inst, bool float_ray) {
     const dv o{of.x, of.y, of.z}, d{df.x, df.y, df.z};
     kind = 0;
     prim = 0;
@@ -426,10 +431,18 @@ void trace64(const scene& s, V of, V df, double t_min, double t_max, double& t_o
     traverse64(s, s.inst_root, o, d, t_min, t_max, [&](uint32_t k, double& tm) {
         const float* in = s.inst + 28 * size_t(k);
         const float* m = in + 12;
-        const dv lo{m[0] * o.x + m[1] * o.y + m[2] * o.z + m[3], m[4] * o.x + m[5] * o.y + m[6] * o.z + m[7],
-                    m[8] * o.x + m[9] * o.y + m[10] * o.z + m[11]};
-        const dv ld{m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
-                    m[8] * d.x + m[9] * d.y + m[10] * d.z};
+        dv lo{m[0] * o.x + m[1] * o.y + m[2] * o.z + m[3], m[4] * o.x + m[5] * o.y + m[6] * o.z + m[7],
+              m[8] * o.x + m[9] * o.y + m[10] * o.z + m[11]};
+        dv ld{m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+              m[8] * d.x + m[9] * d.y + m[10] * d.z};
+        if (float_ray) {
+            const V fo{m[0] * of.x + m[1] * of.y + m[2] * of.z + m[3], m[4] * of.x + m[5] * of.y + m[6] * of.z + m[7],
+                       m[8] * of.x + m[9] * of.y + m[10] * of.z + m[11]};
+            const V fd{m[0] * df.x + m[1] * df.y + m[2] * df.z, m[4] * df.x + m[5] * df.y + m[6] * df.z,
+                       m[8] * df.x + m[9] * df.y + m[10] * df.z};
+            lo = {fo.x, fo.y, fo.z};
+            ld = {fd.x, fd.y, fd.z};
+        }
         traverse64(s, ub(in[24]), lo, ld, t_min, tm, [&](uint32_t q, double& tq) {
             if (tri64(s.tri + 28 * size_t(q), lo, ld, t_min, tq)) {
                 kind = 3;
@@ -754,14 +767,15 @@ void or_trace(const uint32_t* ptgp, const float* rays, size_t n, int any, float*
 }
 
 void or_trace_f64(const uint32_t* ptgp, const float* rays, size_t n, double* t, uint32_t* kind, uint32_t* prim,
-                  uint32_t* inst, int threads) {
+                  uint32_t* inst, int threads, int float_ray) {
     const scene s(ptgp);
     std::atomic<size_t> next{0};
     auto worker = [&] {
         for (size_t b = next.fetch_add(256); b < n; b = next.fetch_add(256))
             for (size_t i = b; i < std::min(n, b + 256); ++i) {
                 const float* q = rays + 8 * i;
-                trace64(s, {q[0], q[1], q[2]}, {q[4], q[5], q[6]}, q[3], q[7], t[i], kind[i], prim[i], inst[i]);
+                trace64(s, {q[0], q[1], q[2]}, {q[4], q[5], q[6]}, q[3], q[7], t[i], kind[i], prim[i], inst[i],
+                        float_ray != 0);
             }
     };
     std::vector<std::thread> pool;

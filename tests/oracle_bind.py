@@ -27,7 +27,7 @@ def oracle():
         lib = ctypes.CDLL(ORACLE_PATH)
         lib.or_trace.argtypes = [_vp, _vp, _sz, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
         lib.or_trace.restype = None
-        lib.or_trace_f64.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp, _vp, ctypes.c_int]
+        lib.or_trace_f64.argtypes = [_vp, _vp, _sz, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int]
         lib.or_trace_f64.restype = None
         lib.or_sample_2d.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
@@ -95,14 +95,17 @@ def or_trace(ptgp: np.ndarray, rays: np.ndarray, any_hit: bool = False, threads:
     return t, u, v, kind, prim, inst, counts
 
 
-def or_trace_f64(ptgp: np.ndarray, rays: np.ndarray, threads: int = THREADS):
+def or_trace_f64(ptgp: np.ndarray, rays: np.ndarray, threads: int = THREADS, float_ray: bool = False):
+    """Closest hits with every candidate primitive tested in double. float_ray: the ray is moved into an
+    instance's object space in float, as the reference moves it (geometry.hpp:213-217), before the double
+    tests; otherwise that transform is in double too."""
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
     n = len(rays)
     t = np.zeros(n, np.float64)
     kind, prim, inst = (np.zeros(n, np.uint32) for _ in range(3))
     w = np.ascontiguousarray(ptgp, np.uint32)
     oracle().or_trace_f64(w.ctypes.data, rays.ctypes.data, n, t.ctypes.data, kind.ctypes.data, prim.ctypes.data,
-                          inst.ctypes.data, threads)
+                          inst.ctypes.data, threads, 1 if float_ray else 0)
     return t, kind, prim, inst
 
 

@@ -64,3 +64,17 @@ def block_mre(a: np.ndarray, b: np.ndarray, block: int = BLOCK, floor: float = 1
     """Mean over blocks and channels of |A - B| / max(B, floor)."""
     ma, mb = block_means(a, block), block_means(b, block)
     return float(np.mean(np.abs(ma - mb) / np.maximum(mb, floor)))
+
+
+def per_value_agreement(a: np.ndarray, b: np.ndarray, rel: float = 1e-4) -> tuple[float, str]:
+    """Fraction of values of a within rel * max(1, |b|) of b, and a note on
+    the ones that are not (how many, the largest difference and where)."""
+    diff = np.abs(a.astype(np.float64) - b.astype(np.float64))
+    tol = rel * np.maximum(1.0, np.abs(b.astype(np.float64)))
+    close = diff <= tol
+    bad = np.argwhere(~close)
+    if len(bad) == 0:
+        return 1.0, "all values agree"
+    worst = tuple(int(x) for x in np.unravel_index(np.argmax(diff - tol), diff.shape))
+    return float(close.mean()), (f"{len(bad)} of {diff.size} values differ, the largest by {diff[worst]:.4g} at "
+                                 f"{worst} ({float(a[worst]):.6g} against {float(b[worst]):.6g})")

@@ -123,9 +123,9 @@ def test_batch_job_matches_oracle():
     with pt.Batch(sc) as b:
         (_, img), st = b.render([pt.BatchJob(0, 40, 30, 4, seed=1), job])
     orgb, _, _ = ob.or_render(sc[3], 96, 54, 8, seed=11)
-    close = np.abs(img - orgb) <= 1e-4 * np.maximum(1.0, np.abs(orgb))
-    print(f"{close.mean():.5f} of values within 1e-4", flush=True)
-    assert close.mean() >= 0.97
+    frac, note = rc.per_value_agreement(img, orgb)
+    print(f"{frac:.5f} of values within 1e-4 ({note})", flush=True)
+    assert frac >= 0.999
 
 
 def test_autotune_then_render(singles):

@@ -20,17 +20,18 @@ def scene_of(name):
 @pytest.mark.parametrize("name", ["materials", "field", "room", "teapot", "crowd", "config0", "config1", "config2",
                                   "config4"])
 def test_render_matches_oracle_per_pixel(name):
-    """Same per-path random streams and the same arithmetic: almost every
-    pixel equals the oracle's to float rounding; a few paths may part where
-    sin/cos round differently (tolerance: 1e-4 relative on 97% of values)."""
+    """Same per-path random streams and the same arithmetic: the pixels equal
+    the oracle's to float rounding, except where a path parts because the
+    device's sinf/cosf round a sample direction differently from the host's
+    libm (tolerance: 1e-4 relative on 99.9% of values; the others printed)."""
     ptgp = scene_of(name)
     W, H, spp = 96, 54, 8
     with pt.Scene(ptgp) as s:
         rgb, _, st = s.render(W, H, spp, seed=11)
     orgb, _, orays = ob.or_render(ptgp, W, H, spp, seed=11)
-    close = np.abs(rgb - orgb) <= 1e-4 * np.maximum(1.0, np.abs(orgb))
-    print(f"{name}: {close.mean():.5f} of values within 1e-4; rays {st.rays} vs oracle {orays}", flush=True)
-    assert close.mean() >= 0.97
+    frac, note = rc.per_value_agreement(rgb, orgb)
+    print(f"{name}: {frac:.5f} of values within 1e-4 ({note}); rays {st.rays} vs oracle {orays}", flush=True)
+    assert frac >= 0.999
     assert abs(st.rays - orays) <= 0.002 * orays + 10
     assert abs(rgb.mean() / max(orgb.mean(), 1e-9) - 1) < 0.02
 
@@ -109,8 +110,9 @@ def test_aux_matches_oracle():
         rgb2, _, _ = s.render(64, 36, 4, seed=8)
     _, oaux, _ = ob.or_render(ptgp, 64, 36, 4, seed=8, aux=True)
     assert np.array_equal(rgb, rgb2)
-    close = np.abs(aux - oaux) <= 1e-4 * np.maximum(1.0, np.abs(oaux))
-    assert close.mean() >= 0.97
+    frac, note = rc.per_value_agreement(aux, oaux)
+    print(f"aux: {frac:.5f} of values within 1e-4 ({note})", flush=True)
+    assert frac >= 0.999
 
 
 def test_no_nee_matches_oracle():
@@ -118,7 +120,9 @@ def test_no_nee_matches_oracle():
     with pt.Scene(ptgp) as s:
         rgb, _, st = s.render(64, 36, 8, seed=3, nee=False)
     orgb, _, orays = ob.or_render(ptgp, 64, 36, 8, seed=3, nee=False)
-    assert (np.abs(rgb - orgb) <= 1e-4 * np.maximum(1.0, np.abs(orgb))).mean() >= 0.97
+    frac, note = rc.per_value_agreement(rgb, orgb)
+    print(f"no NEE: {frac:.5f} of values within 1e-4 ({note})", flush=True)
+    assert frac >= 0.999
     assert st.shadow_rays == 0
 
 
