@@ -10,17 +10,26 @@ query and one per light sample that reaches the occlusion test
 (renderer.hpp:19,72,110).
 
 Besides the timed steps it reports:
-  roofline      the closest-hit traversal stage (the dominant kernel): its
-                algorithmic bytes (per query 32 in + 16 out, 128 per node,
-                4 per leaf reference, 48 per triangle, 16 per sphere, 52 per
-                instance, 16 per large sphere; visit counts from a counting
-                render of the same rays) over its HIP-event time per launch;
-                "traffic" is the HBM bytes per launch that rocprofv3's
-                FETCH_SIZE and WRITE_SIZE counters read for the same launches
-                (profiles/, default workload only);
+  roofline      the traversal stage (the dominant kernel) against HBM:
+                its algorithmic bytes are what a launch must move through
+                HBM at least: per query the ray in and the hit out (32 + 16
+                bytes), and the scene's node and leaf arrays once (the
+                per-visit node, leaf and primitive reads are served by L2 and
+                the 256 MB Infinity Cache, which holds the scene; they are
+                reported apart as bytes served), over its HIP-event time per
+                launch. "traffic" is the HBM bytes per launch that rocprofv3's
+                FETCH_SIZE and WRITE_SIZE counters read for the same launches,
+                measured in this run by child rocprofv3 passes over a
+                one-frame probe (scaled by what they read for a 1 GiB copy),
+                and "issue" the VALU busy fraction and lane use from an SQ
+                counter pass: the limit the kernel actually meets;
   cpu_baseline  the reference's own renderer (oracle/_ref, its headers
                 compiled with its flags) on the host cores, on a bounded
-                sample of the same frame.
+                sample of the same frame, best of three runs; the CPU model
+                and sockets are reported.
+  configs       the sphere field (640x360, 16 spp) and the instancing scene
+                (4096 instances of 50k/200k/500k-triangle meshes, 1080p,
+                32 spp), each against the reference renderer.
   batch         the batched workload: 64 seeded jobs (sphere fields of
                 50-2000 spheres or displaced meshes of 10k-1M triangles;
                 256^2 to 1024^2; 4, 16 or 64 spp) in one pt_render_batch call
@@ -46,31 +55,132 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s (1 GPU); equal-time relMSE guided vs plain NEE, 1080p 1M tris"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per traversal-stage launch on the default workload, measured by
-# two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over this script; the
-# file records the commands and the copy-kernel calibration of each counter.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_extend.icosphere_1M_1080p_64spp.hbm_traffic.txt")
-
-
-def measured_traffic():
-    """hbm_bytes_per_launch from the counter summary, or None without one."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            for line in f:
-                if line.startswith("hbm_bytes_per_launch="):
-                    return int(line.split("=", 1)[1])
-    except OSError:
-        pass
-    return None
+# The traversal stage's kernel, as rocprofv3 names its dispatches.
+EXTEND_KERNEL = "ptd::k_extend<"
+COPY_KERNEL = "copyBuffer"
+SIMDS_PER_CU = 4
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ptsd, width, height, spp_full, seconds):
+def traversal_scene_bytes(ptgp):
+    """Bytes of the scene arrays the traversal reads: the 4-wide nodes, the
+    leaf records (48 bytes per primitive reference) and the instances."""
+    nodes, order, inst = (int(ptgp[3 + 2 * k]) for k in (0, 1, 4))
+    return nodes + 48 * (order // 4) + inst
+
+
+def counter_passes(args, seconds=150):
+    """Live counters for the traversal stage: three child rocprofv3 runs of
+    this script's one-frame probe (the default frame, then a 1 GiB device
+    copy of known traffic), one counter set each (FETCH_SIZE; WRITE_SIZE; SQ
+    VALU counters with GRBM_GUI_ACTIVE), each under its own time limit.
+    Returns per-kernel sums {counter: {kernel: (value, dispatches)}}, or None
+    where rocprofv3 is missing or a pass fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None
+    passes = (("fetch", ["FETCH_SIZE"]), ("write", ["WRITE_SIZE"]),
+              ("valu", ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "GRBM_GUI_ACTIVE"]))
+    env = dict(os.environ, TMPDIR="/tmp")
+    out = {}
+    for name, counters in passes:
+        d = tempfile.mkdtemp(prefix=f"pt_counters_{name}_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", str(seconds), exe, "--pmc", *counters, "--output-format", "csv", "-d", d,
+               "-o", name, "--", sys.executable, os.path.abspath(__file__), "--only", "probe",
+               "--width", str(args.width), "--height", str(args.height), "--spp", str(args.spp),
+               "--scale", str(args.scale), "--stack", str(args.stack), "--copy-probe", "1024"]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        log(f"counters {name}: rocprofv3 exit {r.returncode} in {time.perf_counter() - t0:.1f} s, {len(files)} files")
+        if r.returncode != 0 or not files:
+            log(r.stdout[-2000:])
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        for f in files:
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    kern, cname = row.get("Kernel_Name", ""), row.get("Counter_Name", "")
+                    val = float(row.get("Counter_Value", 0) or 0)
+                    acc = out.setdefault(cname, {}).setdefault(kern, [0.0, 0])
+                    acc[0] += val
+                    acc[1] += 1
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def kernel_sum(counts, counter, pattern):
+    """(sum, dispatches) of a counter over the kernels whose name holds pattern."""
+    total, n = 0.0, 0
+    for kern, (v, k) in counts.get(counter, {}).items():
+        if pattern in kern:
+            total += v
+            n += k
+    return total, n
+
+
+def traversal_counters(counts, cus):
+    """HBM bytes per traversal-stage launch (FETCH_SIZE and WRITE_SIZE, each
+    scaled by what it reads for the 1 GiB copy), and the stage's VALU busy
+    fraction and lane use: busy = 4 cycles per wave64 VALU instruction on a
+    16-lane SIMD over the SIMDs' cycles in the launches (GRBM_GUI_ACTIVE is
+    summed over the 8 XCDs); lane use = VALU thread-cycles over 64 x the
+    waves' VALU cycles."""
+    copy = float(1 << 30)
+    f_copy, _ = kernel_sum(counts, "FETCH_SIZE", COPY_KERNEL)
+    w_copy, _ = kernel_sum(counts, "WRITE_SIZE", COPY_KERNEL)
+    f_ext, n_ext = kernel_sum(counts, "FETCH_SIZE", EXTEND_KERNEL)
+    w_ext, _ = kernel_sum(counts, "WRITE_SIZE", EXTEND_KERNEL)
+    if not (f_copy and w_copy and n_ext):
+        return None
+    f_scale, w_scale = copy / (f_copy * 1024), copy / (w_copy * 1024)
+    traffic = (f_ext * 1024 * f_scale + w_ext * 1024 * w_scale) / n_ext
+    out = {"traffic": int(traffic), "fetch_scale": round(f_scale, 3), "write_scale": round(w_scale, 3),
+           "launches_profiled": n_ext}
+    insts, _ = kernel_sum(counts, "SQ_INSTS_VALU", EXTEND_KERNEL)
+    active, _ = kernel_sum(counts, "SQ_ACTIVE_INST_VALU", EXTEND_KERNEL)
+    thread_cycles, _ = kernel_sum(counts, "SQ_THREAD_CYCLES_VALU", EXTEND_KERNEL)
+    gui, _ = kernel_sum(counts, "GRBM_GUI_ACTIVE", EXTEND_KERNEL)
+    if insts and active and gui:
+        cycles = gui / 8
+        out["valu_busy"] = round(4 * insts / (SIMDS_PER_CU * cus * cycles), 3)
+        out["lane_use"] = round(thread_cycles / (64 * active), 3)
+        out["valu_instructions_per_launch"] = int(insts / n_ext)
+    return out
+
+
+def host_cpu():
+    """The host's CPU model, sockets and logical CPUs (from lscpu)."""
+    import subprocess
+
+    info = {"model": "unknown", "sockets": None, "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "Socket(s)":
+                info["sockets"] = int(v.strip()) if v.strip().isdigit() else v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(ptsd, width, height, spp_full, seconds, repeats=3):
     """The reference renderer on the host cores over a bounded sample of the
-    frame: every row at a reduced spp, sized to about `seconds` of work."""
+    frame: every row at a reduced spp, sized to about `seconds` of work a
+    run; the best of `repeats` runs with the same seed (bench/native.sh's
+    best of three)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
 
@@ -95,17 +205,20 @@ def cpu_baseline(ptsd, width, height, spp_full, seconds):
             return rays, time.perf_counter() - t0
     rays, secs = run(1)
     spp = max(1, min(spp_full, int(seconds / max(secs, 1e-3))))
-    if spp > 1:
-        rays, secs = run(spp)
+    runs = [run(spp) for _ in range(repeats)]
+    rays, secs = min(runs, key=lambda r: r[1])
     if world is not None:
         world.close()
+    cpu = host_cpu()
     return {
         "value": round(rays / secs / 1e6, 3),
         "unit": "Mrays/s",
         "cores": threads,
         "kind": kind,
-        "sample": f"full {width}x{height} frame at {spp} spp (of {spp_full}), seed 1, {rays} rays in {secs:.2f} s, "
-                  f"rows claimed by {threads} threads from an atomic counter as the reference CLI does",
+        "sample": f"full {width}x{height} frame at {spp} spp (of {spp_full}), seed 1, {rays} rays in {secs:.2f} s "
+                  f"(best of {repeats}: {', '.join(f'{r[1]:.2f}' for r in runs)} s), rows claimed by {threads} "
+                  "threads from an atomic counter as the reference CLI does",
+        "cpu_model": cpu["model"], "sockets": cpu["sockets"], "logical_cpus": cpu["logical_cpus"],
     }
 
 
@@ -401,6 +514,23 @@ def guided_kernels(pt):
     return out
 
 
+def probe(pt, args):
+    """One frame of the default workload and a 1 GiB device copy: the
+    program the counter passes profile."""
+    ptgp = pt.pack(pt.generate(1, 1, args.scale))
+    image = pt.DeviceBuffer(args.width * args.height * 12)
+    with pt.Scene(ptgp) as scene:
+        _, _, st = scene.render(args.width, args.height, args.spp, seed=1, device_out=image, stack_entries=args.stack)
+    image.free()
+    nbytes = args.copy_probe << 20
+    if nbytes:
+        a, b = pt.DeviceBuffer(nbytes), pt.DeviceBuffer(nbytes)
+        pt._check(pt.library().pt_device_copy(pt.ctypes.c_void_p(b.ptr), pt.ctypes.c_void_p(a.ptr), nbytes, 2))
+        a.free()
+        b.free()
+    return {"rays": st.rays, "copy_bytes": nbytes}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,7 +541,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--scale", type=float, default=1.0, help="triangle-count scale (1.0 = 1,003,520)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0, help="CPU work per baseline run (best of three)")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--stack", type=int, default=0, help="LDS stack entries per lane (0: the library's choice)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-workload leg")
@@ -424,7 +554,11 @@ def main():
                     help="the equal-time budget: the plain render at this many spp")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoiser leg")
     ap.add_argument("--no-bvh", action="store_true", help="skip the BVH build leg")
-    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels", "configs", "denoise", "bvh"], default=None,
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (HBM traffic and VALU issue of the traversal stage)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the sphere-field and instancing legs")
+    ap.add_argument("--only", choices=["batch", "guided", "guided-kernels", "configs", "denoise", "bvh", "probe"],
+                    default=None,
                     help="run one leg alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also copy MIB MiB device to device (a kernel of known traffic, to scale HBM counters)")
@@ -443,7 +577,8 @@ def main():
                "guided-kernels": lambda: guided_kernels(pt),
                "configs": lambda: configs_leg(pt, args.cpu_seconds),
                "denoise": lambda: denoise_leg(pt),
-               "bvh": lambda: bvh_leg(pt)}[args.only]
+               "bvh": lambda: bvh_leg(pt),
+               "probe": lambda: probe(pt, args)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
 
@@ -488,22 +623,28 @@ def main():
 
     roofline = None
     if not args.no_roofline:
-        # The same rays as step 0 (seed 1): one render counting the closest-hit
-        # stage's visits, one timing each of its launches with HIP events.
+        # The same rays as step 0 (seed 1): one render counting the traversal
+        # stage's visits, one timing each of its launches with HIP events on
+        # the scene's stream.
         _, _, cst = scene.render(W, H, spp, seed=1, count=True, stack_entries=args.stack)
         _, _, tst = scene.render(W, H, spp, seed=1, timing=True, stack_entries=args.stack)
         queries = cst.visits[6]  # closest-hit and shadow queries traced by the traversal-stage launches
-        algo = pt.QUERY_BYTES * queries + sum(v * b for v, b in zip(cst.visits, pt.VISIT_BYTES))
         launches = 10 * tst.batches
-        default_workload = (W, H, spp, args.scale, args.stack) == (1920, 1080, 64, 1.0, 0)
-        traffic = measured_traffic() if default_workload else None
-        achieved = algo / (tst.extend_ms * 1e-3) / 1e9
+        launch_s = tst.extend_ms * 1e-3 / launches
+        served = pt.QUERY_BYTES * queries + sum(v * b for v, b in zip(cst.visits, pt.VISIT_BYTES))
+        scene_bytes = traversal_scene_bytes(ptgp)
+        algo = pt.QUERY_BYTES * queries / launches + scene_bytes   # per launch
+        achieved = algo / launch_s / 1e9
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "k_extend (traversal stage: closest-hit rays + the previous bounce's shadow rays)",
-            "launches": launches, "avg_launch_ms": round(tst.extend_ms / launches, 4),
-            "algorithmic_bytes_per_launch": int(algo / launches),
+            "launches": launches, "avg_launch_ms": round(launch_s * 1e3, 4),
+            "algorithmic_bytes_per_launch": int(algo),
+            "algorithmic_model": f"{pt.QUERY_BYTES} B per query (ray in, hit out) + the scene's nodes, leaf records "
+                                 f"and instances once ({scene_bytes} B)",
+            "bytes_served_per_launch": int(served / launches),
+            "bytes_served_gbs": round(served / launches / launch_s / 1e9, 1),
             "per_query": {"queries": queries, "nodes": cst.visits[0] / queries, "leafrefs": cst.visits[1] / queries,
                           "tris": cst.visits[2] / queries, "spheres": cst.visits[3] / queries},
         }
@@ -513,6 +654,22 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(ptsd, W, H, spp, args.cpu_seconds)
         log(f"cpu baseline: {json.dumps(cpu)}")
+    if roofline is not None and not args.no_counters:
+        # The counter passes run as child processes after the timed frames,
+        # while this process holds its scene but runs nothing on the GPU.
+        counts = counter_passes(args)
+        tc = traversal_counters(counts, cus) if counts else None
+        if tc:
+            roofline["traffic"] = tc["traffic"]
+            roofline["traffic_source"] = ("rocprofv3 FETCH_SIZE and WRITE_SIZE passes in this run (scaled by "
+                                          f"{tc['fetch_scale']} and {tc['write_scale']} from a 1 GiB copy)")
+            roofline["hbm_measured_gbs"] = round(tc["traffic"] / (roofline["avg_launch_ms"] * 1e-3) / 1e9, 1)
+            roofline["hbm_measured_frac"] = round(roofline["hbm_measured_gbs"] / HBM_PEAK_GBS, 4)
+            if "valu_busy" in tc:
+                roofline["issue"] = {"valu_busy": tc["valu_busy"], "lane_use": tc["lane_use"],
+                                     "valu_instructions_per_launch": tc["valu_instructions_per_launch"],
+                                     "source": "rocprofv3 SQ counter pass in this run"}
+        log(f"counters: {json.dumps(tc)}")
     if args.copy_probe:
         nbytes = args.copy_probe << 20
         a, b = pt.DeviceBuffer(nbytes), pt.DeviceBuffer(nbytes)
@@ -527,6 +684,7 @@ def main():
     guided = None if args.no_guided else guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp)
     denoised = None if args.no_denoise else denoise_leg(pt)
     bvh = None if args.no_bvh else bvh_leg(pt)
+    configs = None if args.no_configs else configs_leg(pt, args.cpu_seconds)
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -543,6 +701,7 @@ def main():
         "guided": guided,
         "denoise": denoised,
         "bvh_build": bvh,
+        "configs": configs,
     }
     if cpu:
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
