@@ -187,6 +187,57 @@ struct ptgp_check {
     }
 };
 
+// The device copy of a checked PTGP buffer's nodes, in place in `dev` (a
+// copy of the buffer): the first TOP_NODES nodes met breadth first from the
+// flat root and the instance root get numbers 0, 1, 2 ..., the rest keep
+// their order after them, and every node link (child codes, the instances'
+// mesh roots) follows; each child slot's first and count become one word
+// (first | count << 28, the traversal stack's entry code) in place of
+// `first`, so a node step loads 112 of the node's 128 bytes. Returns the new
+// number of each node.
+std::vector<uint32_t> device_nodes(const ptgp_check& chk, std::vector<uint32_t>& dev) {
+    const uint32_t n = chk.n_nodes;
+    const uint32_t* old = chk.at(0);
+    const float inf = std::numeric_limits<float>::infinity();
+    auto internal_child = [&](uint32_t k, int c) {
+        const float* f = reinterpret_cast<const float*>(old) + 32 * size_t(k);
+        bool empty = true;
+        for (int a = 0; a < 6; ++a) empty = empty && f[4 * a + c] == inf;
+        return !empty && old[32 * size_t(k) + 28 + c] == 0;
+    };
+    std::vector<uint32_t> map(n, NONE), top;
+    std::vector<uint32_t> queue;
+    for (uint32_t r : {chk.flat_root, chk.inst_root})
+        if (r != NONE) queue.push_back(r);
+    for (size_t q = 0; q < queue.size() && top.size() < ptd::TOP_NODES; ++q) {
+        const uint32_t k = queue[q];
+        if (map[k] != NONE) continue;
+        map[k] = static_cast<uint32_t>(top.size());
+        top.push_back(k);
+        for (int c = 0; c < 4; ++c)
+            if (internal_child(k, c)) queue.push_back(old[32 * size_t(k) + 24 + c]);
+    }
+    uint32_t next = static_cast<uint32_t>(top.size());
+    for (uint32_t k = 0; k < n; ++k)
+        if (map[k] == NONE) map[k] = next++;
+    uint32_t* nd = dev.data() + chk.sec[0].offset / 4;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t* to = nd + 32 * size_t(map[k]);
+        std::memcpy(to, old + 32 * size_t(k), 128);
+        for (int c = 0; c < 4; ++c) {
+            uint32_t first = to[24 + c];
+            if (internal_child(k, c)) first = map[first];
+            to[24 + c] = first | (to[28 + c] << 28);
+        }
+    }
+    uint32_t* in = dev.data() + chk.sec[4].offset / 4;
+    for (uint32_t k = 0; k < chk.n_inst; ++k) {
+        uint32_t& root = in[28 * size_t(k) + 24];
+        if (root != NONE) root = map[root];
+    }
+    return map;
+}
+
 int choose_spb(int spp, size_t npix, int requested, size_t free_bytes, size_t extra_path_bytes) {
     if (requested > 0) {
         if (spp % requested) throw pt::status_error(PT_ERR_INVALID, "samples_per_batch must divide spp");
@@ -275,8 +326,10 @@ pt_status pt_scene_create(const uint32_t* ptgp, size_t n_words, pt_scene** out) 
         s->cus = prop.multiProcessorCount;
         HIPC(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         s->host.assign(ptgp, ptgp + n_words);
+        std::vector<uint32_t> dev(ptgp, ptgp + n_words);
+        s->node_map = device_nodes(chk, dev);
         s->words.ensure(n_words * 4);
-        HIPC(hipMemcpy(s->words.p, ptgp, n_words * 4, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s->words.p, dev.data(), n_words * 4, hipMemcpyHostToDevice));
         s->leaf.ensure(chk.leaf.size() * 4);
         if (!chk.leaf.empty()) HIPC(hipMemcpy(s->leaf.p, chk.leaf.data(), chk.leaf.size() * 4, hipMemcpyHostToDevice));
         s->work.ensure(64);
@@ -292,8 +345,11 @@ pt_status pt_scene_create(const uint32_t* ptgp, size_t n_words, pt_scene** out) 
         v.mats = reinterpret_cast<const float4*>(sec(5));
         v.lights = sec(6);
         v.large = sec(7);
-        v.flat_root = chk.flat_root;
-        v.inst_root = chk.inst_root;
+        v.flat_root = s->device_node(chk.flat_root);
+        v.inst_root = s->device_node(chk.inst_root);
+        v.n_cached = 0;
+        v.cache = 0;
+        v.n_top = std::min<uint32_t>(ptd::TOP_NODES, chk.n_nodes);
         v.n_small = chk.n_small;
         v.flags = ptgp[21];
         v.n_lights = chk.n_lights;

@@ -94,6 +94,10 @@ struct pt_scene {
     dbuf words, leaf, spill, work, io_rays, io_hits, counts;
     ptd::SceneView view{};
     int max_stack = 0;   // entries a lane's stack can reach in this scene
+    // New number of each PTGP node in the device copy (the trees' top
+    // levels first); node_map[k] for node k of the caller's buffer.
+    std::vector<uint32_t> node_map;
+    uint32_t device_node(uint32_t k) const { return k == ptd::NONE ? k : node_map[k]; }
     int tuned_stack = 0, tuned_blocks = 0;
     ptapi::render_bufs rb;
     pt_guide* guide = nullptr;   // attached learned guide, if any

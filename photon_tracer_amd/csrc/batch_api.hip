@@ -126,7 +126,7 @@ void check_jobs(const pt_batch* b, const pt_batch_job* jobs, int n_jobs) {
 ptd::SceneView job_view(const pt_batch* b, const pt_batch_job& J) {
     const pt::merge_part& part = b->parts[J.scene];
     ptd::SceneView v = b->scene->view;
-    v.flat_root = part.root;
+    v.flat_root = b->scene->device_node(part.root);
     v.inst_root = NONE;
     v.large = b->scene->view.large + part.large_off;
     v.n_large = part.n_large;

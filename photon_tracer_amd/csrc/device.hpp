@@ -37,9 +37,18 @@ struct SceneView {
     const uint32_t* lights;  // sphere numbers
     const uint32_t* large;   // sphere numbers
     uint32_t flat_root, inst_root, n_small, flags;  // flags: 1 sky, 2 sample lights
-    uint32_t n_lights, n_large, pad0, pad1;
+    uint32_t n_lights, n_large;
+    // Nodes [0, n_cached) are also in the launch's LDS node cache, at LDS
+    // byte address `cache` + 144 * node (0: no cache). The device copy of a
+    // scene numbers the top levels of its trees first, breadth first, so the
+    // nodes every ray visits are the cached ones.
+    uint32_t n_cached, cache;
+    uint32_t n_top;          // nodes the host numbered first (at most TOP_NODES), available for the cache
+    uint32_t pad[3];
     float4 cam[4];           // origin, corner, across, up
 };
+constexpr uint32_t NODE_CACHE_STRIDE = 144;   // bytes per cached node: 7 rows of 16, padded against bank conflicts
+constexpr uint32_t TOP_NODES = 64;            // nodes a scene's device copy numbers first (the cache's upper bound)
 
 struct Hit {
     float t, u, v;
@@ -68,6 +77,21 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b) {
 __device__ __forceinline__ f3 normalize(f3 a) {
     const float len = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
     return {a.x / len, a.y / len, a.z / len};
+}
+
+// 1/x rounded to nearest, as the IEEE division 1.0f / x gives it. The
+// hardware reciprocal refined by one Newton step with FMA is the correctly
+// rounded reciprocal for 2^-126 <= |x| < 2^126: checked against the division
+// for every mantissa and sign at every exponent of that range on gfx950 (the
+// only differences are denormal inputs and results). Zeros, infinities, NaN,
+// denormals and |x| >= 2^126 take the division.
+__device__ __forceinline__ float rcp_rn(float x) {
+    const float ax = fabsf(x);
+    if (__builtin_expect(ax >= 0x1p-126f && ax < 0x1p126f, 1)) {
+        const float r = __builtin_amdgcn_rcpf(x);
+        return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+    }
+    return 1.0f / x;
 }
 
 // ---------------------------------------------------------------------------
@@ -125,7 +149,7 @@ __device__ __forceinline__ bool tri_hit(f3 p0, f3 e1, f3 e2, f3 o, f3 d, float t
     const f3 pvec = cross(d, e2);
     const float det = dot(e1, pvec);
     if (fabsf(det) < 1e-12f) return false;
-    const float inv_det = 1.0f / det;
+    const float inv_det = rcp_rn(det);
     const f3 tvec = o - p0;
     const float u = dot(tvec, pvec) * inv_det;
     if (u < 0 || u > 1) return false;

@@ -19,7 +19,16 @@ struct Ctrl {
     uint32_t w_extend;    // next item of the traversal stage (shadow + closest-hit queues) to hand out
     uint32_t w_unused;
     uint32_t pad[3];
+    uint32_t w_seg[8];    // the traversal stage's work counters, one per segment of its queues (XCD_SEGMENTS)
 };
+// The traversal stage splits its queues into this many contiguous segments,
+// one per XCD: each XCD's waves take work from their own segment (neighbouring
+// pixels' rays, whose nodes then stay in that XCD's L2) and from the others
+// once theirs is done.
+#ifndef PT_XCD_SEGMENTS
+#define PT_XCD_SEGMENTS 8
+#endif
+constexpr uint32_t XCD_SEGMENTS = PT_XCD_SEGMENTS;
 constexpr int MAX_BOUNCES = 64;
 
 // Per-render values the kernels read from HBM rather than from their

@@ -175,22 +175,69 @@ __device__ __forceinline__ void light_sample_arrives(const FrameParams& fp, uint
 // query still goes through the same steps, so the hits are the same.
 template <bool PRIMARY>
 struct refill_threshold {
-    static constexpr uint32_t lanes = PRIMARY ? 64 : 48;
+#ifndef PT_REFILL_LANES
+#define PT_REFILL_LANES 48
+#endif
+    static constexpr uint32_t lanes = PRIMARY ? 64 : PT_REFILL_LANES;
 };
+
+// Nodes of the trees' top levels each traversal block keeps in LDS: 40 x
+// 144 bytes next to the 8-entry stack ring's 16 KB lets 7 blocks of 256
+// share a CU's 160 KB.
+#ifndef PT_NODE_CACHE
+#define PT_NODE_CACHE 40
+#endif
+constexpr uint32_t NODE_CACHE = PT_NODE_CACHE;
+
+#ifndef PT_LEAF_BATCH_P
+#define PT_LEAF_BATCH_P 0
+#endif
+#ifndef PT_LEAF_BATCH_S
+#define PT_LEAF_BATCH_S 12
+#endif
+template <bool PRIMARY>
+struct leaf_batch {
+    static constexpr uint32_t lanes = PRIMARY ? PT_LEAF_BATCH_P : PT_LEAF_BATCH_S;
+};
+
+// Traversal queues of at least this many items per lane of the grid are
+// split into XCD_SEGMENTS segments.
+#ifndef PT_SEGMENT_MIN
+#define PT_SEGMENT_MIN 64
+#endif
+constexpr uint32_t SEGMENT_MIN = PT_SEGMENT_MIN;
 
 // Scenes with instances walk with lane refill too (the instance tree's steps
 // in the same walk); false keeps the per-wave fetch of 64 queries for them.
 constexpr bool INST_REFILL = true;
 
 template <int S, bool COUNT, bool PRIMARY, bool INST, class Stack>
-__device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl* c, uint32_t n_shadow, uint32_t n,
-                                              Stack& st, Visits& vis) {
+__device__ __forceinline__ void extend_refill(const FrameParams& fp, const SceneView& sc, int b, Ctrl* c,
+                                              uint32_t n_shadow, uint32_t n, Stack& st, Visits& vis) {
     const float4* ro = fp.ro[b & 1];
     const float4* rd = fp.rd[b & 1];
     FrameConst fc;
     if (PRIMARY) fc = *fp.frame;
     const uint32_t chunk = 64 * fetch_groups(n);
     const unsigned long long below = (1ull << lane_id()) - 1ull;
+    // Segment x of the work: shadow items [ns * x / S, ns * (x + 1) / S), then
+    // closest-hit items n_shadow + [nr * x / S, nr * (x + 1) / S); pool and
+    // pool_end are offsets within the wave's current segment. Blocks are
+    // dispatched to the XCDs in turn, so block b's XCD is b % 8. Queues of
+    // fewer than SEGMENT_MIN items per lane of the grid stay one segment
+    // (measured: a 640x360 frame's camera rays took twice as long split).
+    const uint32_t nr = n - n_shadow;
+    const uint32_t nseg = uint64_t(n) >= uint64_t(SEGMENT_MIN) * gridDim.x * blockDim.x ? XCD_SEGMENTS : 1u;
+    uint32_t seg = blockIdx.x % nseg;
+    uint32_t seg_done = 0;
+    uint32_t s_lo = 0, s_len = 0, r_lo = 0, r_len = 0;
+    auto seg_bounds = [&](uint32_t x) {
+        s_lo = static_cast<uint32_t>(uint64_t(n_shadow) * x / nseg);
+        s_len = static_cast<uint32_t>(uint64_t(n_shadow) * (x + 1) / nseg) - s_lo;
+        r_lo = static_cast<uint32_t>(uint64_t(nr) * x / nseg);
+        r_len = static_cast<uint32_t>(uint64_t(nr) * (x + 1) / nseg) - r_lo;
+    };
+    seg_bounds(seg);
     uint32_t pool = 0, pool_end = 0;   // the wave's run of items not yet started
     bool drained = false;              // the queue has no more items
     bool live = false, any = false;
@@ -202,15 +249,24 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
         if (drained) {
             if (n_idle == 64) break;
         } else if (n_idle >= refill_threshold<PRIMARY>::lanes) {
-            if (pool == pool_end) {
-                pool = wave_fetch(&c->w_extend, chunk);
-                pool_end = pool < n ? min(pool + chunk, n) : pool;
-                drained = pool >= n;
+            while (pool == pool_end && !drained) {
+                const uint32_t len = s_len + r_len;
+                const uint32_t got = wave_fetch(&c->w_seg[seg], chunk);
+                if (got < len) {
+                    pool = got;
+                    pool_end = min(got + chunk, len);
+                } else if (++seg_done == nseg) {
+                    drained = true;
+                } else {
+                    seg = (seg + 1) % nseg;
+                    seg_bounds(seg);
+                }
             }
             if (!live) {
                 const uint32_t r = static_cast<uint32_t>(__popcll(idle & below));
                 if (pool + r < pool_end) {
-                    item = pool + r;
+                    const uint32_t off = pool + r;
+                    item = off < s_len ? s_lo + off : n_shadow + r_lo + (off - s_len);
                     any = !PRIMARY && item < n_shadow;
                     f3 o, d;
                     float tmin = T_MIN, tmax = T_MAX;
@@ -226,7 +282,7 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
                         tmin = a.w;
                         tmax = bb.w;
                     }
-                    live = !walk_begin<COUNT, INST>(fp.sc, w, o, d, tmin, tmax, any, st, &vis);
+                    live = !walk_begin<COUNT, INST>(sc, w, o, d, tmin, tmax, any, st, &vis);
                     if (!live) {   // ended at once: only an any-hit query on a large sphere
                         item |= 0x80000000u;
                     }
@@ -235,7 +291,18 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, int b, Ctrl
             pool = min(pool + n_idle, pool_end);
         }
         bool done = (item & 0x80000000u) != 0;
-        if (live && walk_step<COUNT, INST>(fp.sc, w, any, st, &vis)) {
+        // A lane whose next entry is a leaf waits while fewer than
+        // LEAF_BATCH lanes are at leaves and others still have nodes to
+        // visit, so the leaf tests' loads run for more lanes at once; each
+        // lane's own sequence of steps is unchanged.
+        bool step = live;
+        if (leaf_batch<PRIMARY>::lanes > 0) {
+            const bool at_leaf = live && (w.e.x >> COUNT_SHIFT) != 0;
+            const uint32_t n_leaf = static_cast<uint32_t>(__popcll(__ballot(at_leaf)));
+            const bool nodes_left = __ballot(live && !at_leaf) != 0;
+            if (at_leaf && nodes_left && n_leaf < leaf_batch<PRIMARY>::lanes) step = false;
+        }
+        if (step && walk_step<COUNT, INST>(sc, w, any, st, &vis)) {
             live = false;
             done = true;
         }
@@ -266,6 +333,8 @@ __global__ __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void k_extend(FrameParams fp, int b) {
     if (!PRIMARY && fp.frame->tail_from <= static_cast<uint32_t>(b)) return;  // the tail launch has these rays
     __shared__ uint2 lds[S * TRACE_BLOCK];
+    __shared__ float4 node_cache[NODE_CACHE * (NODE_CACHE_STRIDE / 16)];
+    const SceneView sc = cache_top_nodes(fp.sc, node_cache, NODE_CACHE);
     auto st = lane_stack<S, TRACE_BLOCK>(lds, fp.spill);
     Visits vis;
     Ctrl* c = fp.ctrl + b;
@@ -274,7 +343,7 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void
     const uint32_t n = n_shadow + n_rays;
     if (PRIMARY && blockIdx.x == 0 && threadIdx.x == 0) c->n_rays = fp.P;
     if constexpr (!INST || INST_REFILL) {
-        extend_refill<S, COUNT, PRIMARY, INST>(fp, b, c, n_shadow, n, st, vis);
+        extend_refill<S, COUNT, PRIMARY, INST>(fp, sc, b, c, n_shadow, n, st, vis);
         if (COUNT) add_visits(fp.visits, vis);
         return;
     }

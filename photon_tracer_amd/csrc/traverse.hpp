@@ -94,17 +94,56 @@ constexpr int COUNT_SHIFT = 28;
 // direction component starting exactly on a box face). The nearest hit child,
 // which the reference pushes last and pops at once, is returned in `next`
 // instead of going through the stack; returns whether there is one.
+//
+// Nodes below sc.n_cached come from the launch's LDS copy (the top levels
+// of the trees, which nearly every ray visits), the others from HBM through
+// the caches: the traversal's limit is the vector memory pipeline, seven
+// 16-byte loads per lane and node step. The LDS rows are read by inline
+// assembly that waits for its own loads, so the compiler cannot merge the two
+// kinds of read into one flat load of a selected address.
+typedef float lds_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 as_f4(lds_f4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+
 template <class Stack>
-__device__ __forceinline__ bool visit_node(const Node* __restrict__ nodes, uint32_t idx, f3 o, f3 inv, float tmin,
-                                           float tmax, Stack& st, uint2& next) {
-    const Node* nd = nodes + idx;
-    const float4 lx = nd->lo[0], ly = nd->lo[1], lz = nd->lo[2];
-    const float4 hx = nd->hi[0], hy = nd->hi[1], hz = nd->hi[2];
-    const uint4 fi = nd->first, co = nd->count;
+__device__ __forceinline__ bool visit_node(const SceneView& sc, uint32_t idx, f3 o, f3 inv, float tmin, float tmax,
+                                           Stack& st, uint2& next) {
+    float4 lx, ly, lz, hx, hy, hz;
+    uint4 cd;   // first | count << 28 (the device copy of the nodes)
+    if (idx < sc.n_cached) {
+        const uint32_t a = sc.cache + idx * NODE_CACHE_STRIDE;
+        lds_f4 r0, r1, r2, r3, r4, r5, r6;
+        asm volatile(
+            "ds_read_b128 %0, %7\n\t"
+            "ds_read_b128 %1, %7 offset:16\n\t"
+            "ds_read_b128 %2, %7 offset:32\n\t"
+            "ds_read_b128 %3, %7 offset:48\n\t"
+            "ds_read_b128 %4, %7 offset:64\n\t"
+            "ds_read_b128 %5, %7 offset:80\n\t"
+            "ds_read_b128 %6, %7 offset:96\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6)
+            : "v"(a));
+        lx = as_f4(r0);
+        ly = as_f4(r1);
+        lz = as_f4(r2);
+        hx = as_f4(r3);
+        hy = as_f4(r4);
+        hz = as_f4(r5);
+        cd = make_uint4(__float_as_uint(r6.x), __float_as_uint(r6.y), __float_as_uint(r6.z), __float_as_uint(r6.w));
+    } else {
+        const Node* nd = sc.nodes + idx;
+        lx = nd->lo[0];
+        ly = nd->lo[1];
+        lz = nd->lo[2];
+        hx = nd->hi[0];
+        hy = nd->hi[1];
+        hz = nd->hi[2];
+        cd = nd->first;
+    }
     const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
     const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-    const uint32_t code[4] = {fi.x | (co.x << COUNT_SHIFT), fi.y | (co.y << COUNT_SHIFT), fi.z | (co.z << COUNT_SHIFT),
-                              fi.w | (co.w << COUNT_SHIFT)};
+    const uint32_t code[4] = {cd.x, cd.y, cd.z, cd.w};
     float tn[4];
     bool h[4];
 #pragma unroll
@@ -146,7 +185,7 @@ __device__ __forceinline__ bool visit_node(const Node* __restrict__ nodes, uint3
     return true;
 }
 
-__device__ __forceinline__ f3 safe_rcp3(f3 d) { return {1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
+__device__ __forceinline__ f3 safe_rcp3(f3 d) { return {rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z)}; }
 
 // Returns whether something was hit. With ANY, stops at the first hit
 // (geometry::occluded); otherwise fills `hit` with the nearest.
@@ -186,7 +225,7 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                have = visit_node(sc.nodes, first, o, inv, tmin, tmax, st, e);
+                have = visit_node(sc, first, o, inv, tmin, tmax, st, e);
                 continue;
             }
             for (uint32_t i = first; i < first + count; ++i) {
@@ -259,8 +298,8 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                if (in_blas) have = visit_node(sc.nodes, first, lo, linv, tmin, tmax, st, e);
-                else have = visit_node(sc.nodes, first, o, inv, tmin, tmax, st, e);
+                if (in_blas) have = visit_node(sc, first, lo, linv, tmin, tmax, st, e);
+                else have = visit_node(sc, first, o, inv, tmin, tmax, st, e);
                 continue;
             }
             if (COUNT) vis->leafrefs += count;
@@ -317,9 +356,20 @@ struct Walk : WalkInst<INST> {
     bool have;
 };
 
+// Moves a walk on to the next stack entry it will process: the pending
+// entry, or the ones popped after it, skipping every entry whose entry
+// distance is not below the nearest hit so far (bvh.hpp:74); INST: also the
+// instance tree's bookkeeping between entries (the next instance of a leaf,
+// the switch from the flat tree to the instance tree). These are the steps
+// trace_ray takes between two node or leaf visits, so a walk step always
+// visits a node or a leaf and the step that empties the stack ends the walk.
+// Returns true when the walk has ended.
+template <bool COUNT, bool INST, class Stack>
+__device__ __forceinline__ bool walk_advance(const SceneView& sc, Walk<INST>& w, Stack& st, Visits* vis);
+
 // Starts a walk: the large spheres in double, then the flat tree's root.
 // Returns true if the query has already ended (an any-hit query that hit a
-// large sphere).
+// large sphere, or nothing to walk).
 template <bool COUNT, bool INST = false, class Stack>
 __device__ __forceinline__ bool walk_begin(const SceneView& sc, Walk<INST>& w, f3 o, f3 d, float tmin, float tmax,
                                            bool any, Stack& st, Visits* vis) {
@@ -347,7 +397,7 @@ __device__ __forceinline__ bool walk_begin(const SceneView& sc, Walk<INST>& w, f
     st.reset();
     w.e = make_uint2(sc.flat_root, __float_as_uint(tmin));
     w.have = sc.flat_root != NONE;
-    return false;
+    return walk_advance<COUNT, INST>(sc, w, st, vis);
 }
 
 // Enters instance k of an instance-tree leaf: the ray to its object space
@@ -370,19 +420,18 @@ __device__ __forceinline__ void walk_enter(const SceneView& sc, Walk<true>& w, u
     w.have = root != NONE;
 }
 
-// One stack entry of the walk: a node's slab test, or a leaf's primitives
-// (INST: or the instance tree's bookkeeping: the next instance of a leaf, the
-// switch from the flat tree to the instance tree). Returns true when the walk
-// has ended (every tree walked, or an any-hit query found a hit); kind != 0
-// then says something was hit.
-template <bool COUNT, bool INST = false, class Stack>
-__device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bool any, Stack& st, Visits* vis) {
-    if (!w.have) {
+template <bool COUNT, bool INST, class Stack>
+__device__ __forceinline__ bool walk_advance(const SceneView& sc, Walk<INST>& w, Stack& st, Visits* vis) {
+    while (true) {
+        if (w.have) {
+            if (!(__uint_as_float(w.e.y) >= w.tmax)) return false;
+            w.have = false;   // behind the nearest hit: skipped
+        }
         if constexpr (INST) {
             if (w.in_blas && st.top == w.blas_base) {   // the entered instance's mesh tree is done
                 if (++w.li < w.lend) {
                     walk_enter<COUNT>(sc, w, sc.order[w.li], st, vis);
-                    return false;
+                    continue;
                 }
                 w.in_blas = false;
             }
@@ -392,26 +441,35 @@ __device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bo
                 st.reset();
                 w.e = make_uint2(sc.inst_root, __float_as_uint(w.tmin));
                 w.have = true;
-                return false;
+                continue;
             }
         } else {
             if (st.top == 0) return true;
         }
         w.e = st.pop();
+        w.have = true;
     }
+}
+
+// One node or leaf of the walk (the entry walk_advance left pending): a
+// node's slab test, a leaf's primitives, or (INST) an instance-tree leaf's
+// first instance entered; then walk_advance to the next entry. Returns true
+// when the walk has ended (every tree walked, or an any-hit query found a
+// hit); kind != 0 then says something was hit.
+template <bool COUNT, bool INST = false, class Stack>
+__device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bool any, Stack& st, Visits* vis) {
     w.have = false;
-    if (__uint_as_float(w.e.y) >= w.tmax) return false;
     const uint32_t first = w.e.x & FIRST_MASK, count = w.e.x >> COUNT_SHIFT;
     if (count == 0) {
         if (COUNT) ++vis->nodes;
-        if constexpr (INST) {
-            if (w.in_blas) {
-                w.have = visit_node(sc.nodes, first, w.lo, w.linv, w.tmin, w.tmax, st, w.e);
-                return false;
-            }
+        bool in_blas = false;
+        if constexpr (INST) in_blas = w.in_blas;
+        if (in_blas) {
+            if constexpr (INST) w.have = visit_node(sc, first, w.lo, w.linv, w.tmin, w.tmax, st, w.e);
+        } else {
+            w.have = visit_node(sc, first, w.o, w.inv, w.tmin, w.tmax, st, w.e);
         }
-        w.have = visit_node(sc.nodes, first, w.o, w.inv, w.tmin, w.tmax, st, w.e);
-        return false;
+        return walk_advance<COUNT, INST>(sc, w, st, vis);
     }
     if constexpr (INST) {
         if (w.tlas) {
@@ -420,7 +478,7 @@ __device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bo
                 w.li = first;
                 w.lend = first + count;
                 walk_enter<COUNT>(sc, w, sc.order[first], st, vis);
-                return false;
+                return walk_advance<COUNT, INST>(sc, w, st, vis);
             }
             for (uint32_t i = first; i < first + count; ++i) {   // a mesh leaf, in object space
                 if (COUNT) ++vis->tris;
@@ -432,7 +490,7 @@ __device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bo
                     if (any) return true;
                 }
             }
-            return false;
+            return walk_advance<COUNT, INST>(sc, w, st, vis);
         }
     }
     for (uint32_t i = first; i < first + count; ++i) {
@@ -457,7 +515,25 @@ __device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bo
             if (any) return true;
         }
     }
-    return false;
+    return walk_advance<COUNT, INST>(sc, w, st, vis);
+}
+
+// Copies nodes [0, min(n_cache, n_top)) of the device node array into a block's LDS node
+// cache (7 rows of 16 bytes each, NODE_CACHE_STRIDE apart) and returns the
+// scene view that reads them from there. Called by every thread of the
+// block; ends with a barrier.
+__device__ __forceinline__ SceneView cache_top_nodes(const SceneView& sc, float4* cache, uint32_t n_cache) {
+    SceneView v = sc;
+    const uint32_t n = min(n_cache, sc.n_top);   // the device copy numbers its top nodes first
+    const float4* src = reinterpret_cast<const float4*>(sc.nodes);
+    for (uint32_t i = threadIdx.x; i < 7 * n; i += blockDim.x) {
+        const uint32_t k = i / 7, row = i % 7;
+        cache[k * (NODE_CACHE_STRIDE / 16) + row] = src[8 * k + row];
+    }
+    __syncthreads();
+    v.n_cached = n;
+    v.cache = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) float4*)(cache)));
+    return v;
 }
 
 // One lane's stack over the block's LDS ring array and the grid's spill area.
