@@ -23,7 +23,13 @@
  *   parameters (camera.hpp:18-34). Layout in pt_layout comments below.
  *   PTGP v1 is the reference's packed GPU scene, byte for byte
  *   (gpu_pack.hpp:21-33,77-82,177-203): 48-word header, 8 sections, 16-byte
- *   aligned. pt_pack produces it from PTSD with this library's own binned-SAH
+ *   aligned. The reference drops textures (gpu_pack.hpp:96); this library
+ *   carries them in words the reference writes as zero, so an untextured
+ *   scene's buffer is unchanged: header words 38-40 give the byte offset,
+ *   length and count of a texture section (per texture width, height, first
+ *   texel, 0; then linear RGB texels as 4 floats, rows top first), and word
+ *   9 of a matte material's 12 holds its texture's number + 1 (0: none).
+ *   pt_pack produces PTGP from PTSD with this library's own binned-SAH
  *   4-wide builder, which emits the reference's node layout (bvh.hpp:27-32),
  *   without the reference's 10-level depth cap (gpu_pack.hpp:36,61-62).
  */
@@ -91,11 +97,45 @@ pt_status pt_generate_job(uint64_t seed, double scale, pt_job_desc* desc, uint32
  * line names: usemtl picks its materials, mapped as make_material maps them
  * (obj.hpp:89-112) and appended to the scene's materials; faces before any
  * usemtl, or naming a material the library lacks, take `material` (an index
- * into the scene's materials). Texture maps are not carried: PTGP has none
- * (gpu_pack.hpp:96). Returns a new PTSD buffer (free with pt_free). */
+ * into the scene's materials). Texture maps are left out here (the
+ * materials are untextured, as the reference makes them when its reader
+ * finds no image; pt_ptsd_add_obj_textured takes the images). Returns a new
+ * PTSD buffer (free with pt_free). */
 pt_status pt_ptsd_add_obj(const uint32_t* ptsd, size_t n_words, const char* obj_text, size_t text_len,
                           const char* mtl_text, size_t mtl_len, const float base[3], double height, uint32_t material,
                           double turn_deg, uint32_t** out, size_t* out_words);
+
+/* An 8-bit sRGB image as the reference's image8 holds one (image_io.hpp:
+ * rows top first, 3 bytes a texel), named as an MTL map_Kd line names it. */
+typedef struct {
+    const char* name;
+    uint32_t width, height;     /* 1 .. 65536 */
+    const uint8_t* rgb;         /* width * height * 3 bytes */
+} pt_image;
+
+/* decode_image (image_io.hpp:295): a PNG or PPM file's bytes (told apart by
+ * their first bytes) as 8-bit sRGB, 3 bytes a pixel, rows top first, for
+ * pt_image. PNG: not interlaced, 8 or 16 bits a sample (16-bit samples keep
+ * their high byte), gray, gray + alpha, RGB, RGBA or an 8-bit palette, alpha
+ * dropped (image_io.hpp:185-259); PPM: P3 or P6, maximum value 1 to 255,
+ * scaled to 255 (image_io.hpp:262-291). 1 to 65536 pixels a side. rgb is
+ * allocated by the library (free with pt_free). PT_ERR_INVALID with the
+ * reason for what it cannot read. */
+pt_status pt_decode_image(const void* bytes, size_t n_bytes, uint32_t* width, uint32_t* height, uint8_t** rgb);
+
+/* pt_ptsd_add_obj with texture maps: a matte MTL material whose map_Kd names
+ * one of `images` (by the map_Kd line's last word, the file name the
+ * reference's file reader opens) gets that image as its texture, so its
+ * albedo is Kd times the bilinear, repeating, sRGB-decoded texture at the
+ * hit's texture coordinates (load_obj + make_material, obj.hpp:103-111;
+ * image_texture, texture.hpp:13-32). In place of the reference's file reader
+ * the caller passes the images, decoded (pt_decode_image does what the
+ * reference's decode_image does). Each image used is appended to the scene's
+ * textures once; images no map_Kd names are not added. */
+pt_status pt_ptsd_add_obj_textured(const uint32_t* ptsd, size_t n_words, const char* obj_text, size_t text_len,
+                                   const char* mtl_text, size_t mtl_len, const pt_image* images, size_t n_images,
+                                   const float base[3], double height, uint32_t material, double turn_deg,
+                                   uint32_t** out, size_t* out_words);
 
 /* Builds every BVH of a PTSD scene (binned SAH, 16 bins, max leaf 8, 4-wide
  * collapse; bvh.hpp:39-52,116-247; geometry.hpp:74-104) and packs the scene

@@ -22,13 +22,18 @@
 //                 non-QMC numbers from the per-path counter-based stream the
 //                 GPU path defines (key = f(seed, pixel, sample), splitmix64),
 //                 since the reference's per-thread sequential xoshiro stream
-//                 (vec3.hpp:11-52) cannot be reproduced by a parallel tracer.
+//                 (vec3.hpp:11-52) cannot be reproduced by a parallel tracer;
+//                 textured mattes multiply the albedo by
+//   or_texture_sample  image_texture::sample (texture.hpp:22-33) over the
+//                 PTGP texture section's texels.
 //
-// Pinning (tests/test_oracle.py): or_trace against the reference's own
-// traversal (oracle/_ref) bit for bit, and against or_trace_f64;
-// or_sample_2d against the reference sampler; or_render against the
-// reference renderer statistically, against closed forms (white furnace,
-// one sphere light) and against the reference's own render fixtures.
+// Pinning (tests/test_oracle.py, tests/test_textures.py): or_trace against
+// the reference's own traversal (oracle/_ref) bit for bit, and against
+// or_trace_f64; or_sample_2d against the reference sampler;
+// or_texture_sample against the reference's image_texture bit for bit;
+// or_render against the reference renderer statistically, against closed
+// forms (white furnace, one sphere light) and against the reference's own
+// render fixtures (the textured "spot crowd" among them).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -85,6 +90,9 @@ struct scene {
     const uint32_t* large;
     uint32_t n_nodes, n_order, n_sph, n_tri, n_inst, n_mat, n_lights, n_large;
     uint32_t flat_root, inst_root, n_small, flags;
+    const uint32_t* tex_info = nullptr;   // per texture: width, height, first texel, 0
+    const float* texels = nullptr;        // linear RGB + 0 per texel
+    uint32_t n_tex = 0;
     V cam[4];
 
     explicit scene(const uint32_t* words) : w(words) {
@@ -112,6 +120,11 @@ struct scene {
         n_small = words[20];
         flags = words[21];
         for (int r = 0; r < 4; ++r) cam[r] = {fb(words[22 + 4 * r]), fb(words[23 + 4 * r]), fb(words[24 + 4 * r])};
+        n_tex = words[40];
+        if (n_tex) {
+            tex_info = words + words[38] / 4;
+            texels = reinterpret_cast<const float*>(tex_info + 4 * n_tex);
+        }
     }
 };
 
@@ -317,6 +330,34 @@ bool trace(const scene& s, V o, V d, float t_min, float t_max, bool any, hit_out
     if (any) return false;
     h.t = t_max;
     return h.kind != 0;
+}
+
+// image_texture::sample (texture.hpp:22-33): bilinear lookup at (u, v),
+// repeating, v up the image whose rows are stored top first, over the linear
+// texels the packer made with image_texture's table (texture.hpp:13-20).
+V texture_sample(const scene& s, uint32_t t, float u, float v) {
+    const uint32_t* info = s.tex_info + 4 * t;
+    const int w = static_cast<int>(info[0]), h = static_cast<int>(info[1]);
+    const float* tx = s.texels + 4 * size_t(info[2]);
+    const float x = (u - std::floor(u)) * w - 0.5f;
+    const float y = (1 - (v - std::floor(v))) * h - 0.5f;
+    // A NaN or infinite coordinate reads texel 0 with NaN weights (finite
+    // ones put x in [-0.5, w - 0.5]); the reference converts NaN to int with
+    // its platform's result and multiplies by the same NaN weights.
+    auto cell = [](float c) {
+        const float f = std::floor(c);
+        return f >= -1.0f && f <= 65536.0f ? static_cast<int>(f) : 0;
+    };
+    const int x0 = cell(x), y0 = cell(y);
+    const float fx = x - x0, fy = y - y0;
+    auto texel = [&](int xi, int yi) {
+        xi = ((xi % w) + w) % w;
+        yi = ((yi % h) + h) % h;
+        const float* c = tx + 4 * (size_t(yi) * w + xi);
+        return V{c[0], c[1], c[2]};
+    };
+    return texel(x0, y0) * ((1 - fx) * (1 - fy)) + texel(x0 + 1, y0) * (fx * (1 - fy)) +
+           texel(x0, y0 + 1) * ((1 - fx) * fy) + texel(x0 + 1, y0 + 1) * (fx * fy);
 }
 
 // ---- FP64 closest hit: every candidate primitive in double -----------------
@@ -580,6 +621,7 @@ sample_result path(const scene& s, V o, V d, int depth, bool nee, uint32_t pseed
         const V p = o + d * h.t;
         V n;
         bool front;
+        float tex_u = 0, tex_v = 0;   // hittable.hpp:14: 0 unless a triangle sets them
         uint32_t m;
         bool sampled_light = false;
         if (h.kind <= 2) {
@@ -597,6 +639,10 @@ sample_result path(const scene& s, V o, V d, int depth, bool nee, uint32_t pseed
             m = ub(t[3]);
             const V n0{t[12], t[13], t[14]}, n1{t[16], t[17], t[18]}, n2{t[20], t[21], t[22]}, fn{t[24], t[25], t[26]};
             const V sn = unit(n0 * (1 - h.u - h.v) + n1 * h.u + n2 * h.v);
+            // triangle::fill's texture coordinates (triangle.hpp:72-73); the
+            // packed uv sit in the w words of rows 1-6 (gpu_pack.hpp:116-129)
+            tex_u = t[7] * (1 - h.u - h.v) + t[15] * h.u + t[23] * h.v;
+            tex_v = t[11] * (1 - h.u - h.v) + t[19] * h.u + t[27] * h.v;
             if (h.inst == NONE) {
                 front = dot(d, fn) < 0;
                 n = front ? sn : -sn;
@@ -617,6 +663,8 @@ sample_result path(const scene& s, V o, V d, int depth, bool nee, uint32_t pseed
         const int type = static_cast<int>(M[3]);
         if (front && !(sampled && sampled_light)) rad = rad + mul(thr, emission);
         V att = albedo, dir{0, 0, 0};
+        const uint32_t tex = ub(M[9]);
+        if (type == 0 && tex != 0) att = mul(att, texture_sample(s, tex - 1, tex_u, tex_v));   // material.hpp:72
         bool ok = true;
         if (type == 0) {
             float a, b;
@@ -781,6 +829,18 @@ void or_trace_f64(const uint32_t* ptgp, const float* rays, size_t n, double* t, 
     std::vector<std::thread> pool;
     for (int k = 0; k < std::max(1, threads); ++k) pool.emplace_back(worker);
     for (auto& th : pool) th.join();
+}
+
+// image_texture::sample of texture t of a PTGP buffer's texture section at
+// n (u, v) pairs: 3 floats each.
+void or_texture_sample(const uint32_t* ptgp, uint32_t t, const float* uv, size_t n, float* out) {
+    const scene s(ptgp);
+    for (size_t i = 0; i < n; ++i) {
+        const V c = texture_sample(s, t, uv[2 * i], uv[2 * i + 1]);
+        out[3 * i] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+    }
 }
 
 void or_sample_2d(uint64_t seed, int i, int j, uint32_t index, int pair, float* a, float* b) {

@@ -65,6 +65,24 @@ ref_world* from_ptsd(const uint32_t* w, size_t n) {
     const point3 from(fb(w[3]), fb(w[4]), fb(w[5])), at(fb(w[6]), fb(w[7]), fb(w[8]));
     rw->cam = camera(from, at, static_cast<double>(fb(w[9])));
     const uint32_t n_mat = w[10], n_sph = w[11], n_large = w[12], n_tri = w[13], n_mesh = w[14], n_inst = w[15];
+    // The textures (PTSD word 17 of them) follow the instances: the reference's
+    // image_texture made from each image8, as its OBJ loader makes them.
+    std::vector<std::shared_ptr<const image_texture>> textures;
+    {
+        size_t t = 32 + 12 * size_t(n_mat) + 5 * size_t(n_sph + n_large) + 28 * size_t(n_tri) + n_mesh +
+                   28 * size_t(w[16]) + 16 * size_t(n_inst);
+        for (uint32_t i = 0; i < w[17]; ++i) {
+            if (t + 2 > n) throw std::runtime_error("PTSD textures truncated");
+            image8 img;
+            img.width = static_cast<int>(w[t]);
+            img.height = static_cast<int>(w[t + 1]);
+            img.rgb.resize(size_t(img.width) * img.height * 3);
+            if (t + 2 + (img.rgb.size() + 3) / 4 > n) throw std::runtime_error("PTSD textures truncated");
+            std::memcpy(img.rgb.data(), w + t + 2, img.rgb.size());
+            t += 2 + (img.rgb.size() + 3) / 4;
+            textures.push_back(std::make_shared<image_texture>(img));
+        }
+    }
     size_t k = 32;
     std::vector<const material*> mats;
     for (uint32_t m = 0; m < n_mat; ++m, k += 12) {
@@ -72,7 +90,8 @@ ref_world* from_ptsd(const uint32_t* w, size_t n) {
         const uint32_t type = w[k + 3];
         const color emission(fb(w[k + 4]), fb(w[k + 5]), fb(w[k + 6]));
         const float fuzz = fb(w[k + 7]), ior = fb(w[k + 8]);
-        if (type == 0) mats.push_back(g.own(std::make_shared<lambertian>(albedo)));
+        const uint32_t tex = w[k + 9];
+        if (type == 0) mats.push_back(g.own(std::make_shared<lambertian>(albedo, tex ? textures.at(tex - 1) : nullptr)));
         else if (type == 1) mats.push_back(g.own(std::make_shared<metal>(albedo, fuzz)));
         else if (type == 2) mats.push_back(g.own(std::make_shared<dielectric>(ior)));
         else mats.push_back(g.own(std::make_shared<diffuse_light>(emission)));
@@ -106,6 +125,7 @@ ref_world* from_ptsd(const uint32_t* w, size_t n) {
         a.t = vec3(fb(w[k + 11]), fb(w[k + 12]), fb(w[k + 13]));
         g.add_instance(w[k], a, w[k + 1] == ~0u ? nullptr : mats.at(w[k + 1]));
     }
+    for (const auto& tex : textures) k += 2 + (tex->source.rgb.size() + 3) / 4;
     if (k != n) throw std::runtime_error("PTSD size mismatch");
     g.build(true);
     return rw;
@@ -126,7 +146,7 @@ void camera_params(scene_id id, float from[3], float at[3], float* vfov) {
     *vfov = x.v;
 }
 
-geometry reference_scene(int id, int crowd_side, int flatten, const char* obj_path) {
+geometry reference_scene(int id, int crowd_side, int flatten, const char* obj_path, double turn_deg) {
     std::string text;
     std::string dir;
     if (obj_path && *obj_path) {
@@ -145,7 +165,7 @@ geometry reference_scene(int id, int crowd_side, int flatten, const char* obj_pa
         b << f.rdbuf();
         return b.str();
     };
-    return build_world(static_cast<scene_id>(id), true, text, read, 0, crowd_side, flatten != 0);
+    return build_world(static_cast<scene_id>(id), true, text, read, turn_deg, crowd_side, flatten != 0);
 }
 
 // The reference geometry's flat arrays as PTSD.
@@ -210,13 +230,23 @@ std::vector<uint32_t> to_ptsd(const geometry& g, const float from[3], const floa
     out[14] = static_cast<uint32_t>(g.meshes.size());
     out[15] = static_cast<uint32_t>(g.instances.size());
     out[16] = static_cast<uint32_t>(mesh_tris.size() / 28);
+    std::map<const image_texture*, uint32_t> tex_index;   // texture number + 1
+    std::vector<const image_texture*> tex_order;
     for (const material* m : order) {
         color albedo(0, 0, 0);
-        uint32_t type = 3;
+        uint32_t type = 3, tex = 0;
         float fuzz = 0, ior = 1.5f;
         if (auto* l = dynamic_cast<const lambertian*>(m)) {
             type = 0;
             albedo = l->albedo;
+            if (l->texture) {
+                auto it = tex_index.find(l->texture.get());
+                if (it == tex_index.end()) {
+                    tex_order.push_back(l->texture.get());
+                    it = tex_index.emplace(l->texture.get(), static_cast<uint32_t>(tex_order.size())).first;
+                }
+                tex = it->second;
+            }
         } else if (auto* me = dynamic_cast<const metal*>(m)) {
             type = 1;
             albedo = me->albedo;
@@ -230,11 +260,20 @@ std::vector<uint32_t> to_ptsd(const geometry& g, const float from[3], const floa
         for (int c = 0; c < 3; ++c) put(m->emission.e[c]);
         put(fuzz);
         put(ior);
-        body.push_back(0);
+        body.push_back(tex);
         body.push_back(0);
         body.push_back(0);
     }
     for (auto* part : {&sph, &lrg, &tri, &mesh_sizes, &mesh_tris, &inst}) body.insert(body.end(), part->begin(), part->end());
+    for (const image_texture* t : tex_order) {   // the decoded image, as image_texture keeps it (texture.hpp:36-37)
+        const image8& img = t->source;
+        body.push_back(static_cast<uint32_t>(img.width));
+        body.push_back(static_cast<uint32_t>(img.height));
+        const size_t at = body.size();
+        body.resize(at + (img.rgb.size() + 3) / 4, 0);
+        std::memcpy(&body[at], img.rgb.data(), img.rgb.size());
+    }
+    out[17] = static_cast<uint32_t>(tex_order.size());
     out.insert(out.end(), body.begin(), body.end());
     return out;
 }
@@ -355,10 +394,10 @@ void ref_world_camera(void* w, float* out) {
 
 // One of the reference's scenes (0 materials, 1 field, 2 mesh, 3 room,
 // 4 crowd) with its camera, as PTSD, and packed by the reference as PTGP.
-int ref_scene(int id, int crowd_side, int flatten, const char* obj_path, uint32_t** ptsd, size_t* n_ptsd,
-              uint32_t** ptgp, size_t* n_ptgp) {
+int ref_scene(int id, int crowd_side, int flatten, const char* obj_path, double turn_deg, uint32_t** ptsd,
+              size_t* n_ptsd, uint32_t** ptgp, size_t* n_ptgp) {
     try {
-        const geometry g = reference_scene(id, crowd_side, flatten, obj_path);
+        const geometry g = reference_scene(id, crowd_side, flatten, obj_path, turn_deg);
         float from[3], at[3], vfov;
         camera_params(static_cast<scene_id>(id), from, at, &vfov);
         *ptsd = copy_out(to_ptsd(g, from, at, vfov), n_ptsd);
@@ -456,6 +495,38 @@ int ref_sample_2d(uint64_t seed, int i, int j, uint32_t index, int pair, float* 
         sample_2d(off, *a, *b);
     }
     return pair > 2 * qmc_bounces ? 1 : 0;
+}
+
+// The reference's image_texture (texture.hpp:13-33) over w x h sRGB bytes,
+// sampled at n (u, v) pairs: 3 floats each.
+void ref_texture_sample(const uint8_t* rgb, int w, int h, const float* uv, size_t n, float* out) {
+    image8 img;
+    img.width = w;
+    img.height = h;
+    img.rgb.assign(rgb, rgb + 3 * static_cast<size_t>(w) * h);
+    const image_texture tex(img);
+    for (size_t i = 0; i < n; ++i) {
+        const color c = tex.sample(uv[2 * i], uv[2 * i + 1]);
+        out[3 * i] = c.x();
+        out[3 * i + 1] = c.y();
+        out[3 * i + 2] = c.z();
+    }
+}
+
+// decode_image (image_io.hpp:295) of a file's bytes: 0 and the image (rgb
+// allocated with malloc, free with ref_free), or 1 and the error message.
+int ref_decode_image(const uint8_t* bytes, size_t n, int* w, int* h, uint8_t** rgb) {
+    try {
+        const image8 img = decode_image(std::string(reinterpret_cast<const char*>(bytes), n));
+        *w = img.width;
+        *h = img.height;
+        *rgb = static_cast<uint8_t*>(std::malloc(std::max<size_t>(1, img.rgb.size())));
+        if (!img.rgb.empty()) std::memcpy(*rgb, img.rgb.data(), img.rgb.size());
+        return 0;
+    } catch (const std::exception& e) {
+        last_error = e.what();
+        return 1;
+    }
 }
 
 // std::thread::hardware_concurrency(), as main.cpp:116-118 uses.

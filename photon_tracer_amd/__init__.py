@@ -111,6 +111,11 @@ class _TuneResult(ctypes.Structure):
                 ("n_tried", ctypes.c_int32), ("max_paths", ctypes.c_uint64), ("tried", _Tried * 32)]
 
 
+class _Image(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("rgb", ctypes.c_void_p)]
+
+
 class _JobDesc(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_int32),
@@ -144,6 +149,11 @@ def library() -> ctypes.CDLL:
     lib.pt_ptsd_add_obj.argtypes = [vp, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_float),
                                     ctypes.c_double, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(_u32p),
                                     ctypes.POINTER(sz)]
+    lib.pt_ptsd_add_obj_textured.argtypes = [vp, sz, ctypes.c_char_p, sz, ctypes.c_char_p, sz, vp, sz,
+                                             ctypes.POINTER(ctypes.c_float), ctypes.c_double, ctypes.c_uint32,
+                                             ctypes.c_double, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
+    lib.pt_decode_image.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))]
     lib.pt_pack_ex.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz)]
     lib.pt_bvh_build.argtypes = [vp, sz, ctypes.c_int, ctypes.POINTER(_u32p), ctypes.POINTER(sz), ctypes.POINTER(_u32p),
                                  ctypes.POINTER(ctypes.c_double)]
@@ -185,7 +195,7 @@ def library() -> ctypes.CDLL:
     for name in ("pt_batch_merge", "pt_batch_create", "pt_batch_destroy", "pt_render_batch", "pt_batch_autotune"):
         getattr(lib, name).restype = ctypes.c_int
     for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate",
-                 "pt_generate_job", "pt_pack", "pt_pack_ex", "pt_bvh_build", "pt_ptsd_add_obj", "pt_ptgp_depth", "pt_scene_create",
+                 "pt_generate_job", "pt_pack", "pt_pack_ex", "pt_bvh_build", "pt_ptsd_add_obj", "pt_ptsd_add_obj_textured", "pt_decode_image", "pt_ptgp_depth", "pt_scene_create",
                  "pt_scene_destroy", "pt_trace",
                  "pt_trace_device", "pt_trace_device_noalloc", "pt_render", "pt_guide_create", "pt_guide_destroy", "pt_scene_set_guide",
                  "pt_guide_get_weights", "pt_guide_set_weights", "pt_guide_eval", "pt_guide_pdf", "pt_guide_sample",
@@ -264,17 +274,45 @@ def pack(ptsd, device_bvh: bool = False) -> np.ndarray:
     return _take_words(p, n)
 
 
-def add_obj(ptsd, obj_text, base, height: float, material: int, turn_deg: float = 0.0, mtl_text=None) -> np.ndarray:
+def decode_image(data: bytes) -> np.ndarray:
+    """decode_image: a PNG or PPM file's bytes as an (h, w, 3) uint8 sRGB array, rows top first."""
+    data = bytes(data)
+    w, h, p = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.POINTER(ctypes.c_uint8)()
+    _check(library().pt_decode_image(data, len(data), ctypes.byref(w), ctypes.byref(h), ctypes.byref(p)))
+    try:
+        n = 3 * w.value * h.value
+        return np.ctypeslib.as_array(p, shape=(n,)).copy().reshape(h.value, w.value, 3)
+    finally:
+        library().pt_free(p)
+
+
+def add_obj(ptsd, obj_text, base, height: float, material: int, turn_deg: float = 0.0, mtl_text=None,
+            images=None) -> np.ndarray:
     """load_obj: the OBJ text's triangles appended to a PTSD scene, with
     materials from mtl_text (the library every mtllib names) or the given
-    material index; returns the new PTSD words."""
+    material index; images: {name: image} for the MTL's map_Kd names
+    (textures of matte materials), each image an (h, w, 3) uint8 sRGB array
+    or a PNG/PPM file's bytes (decode_image). Returns the new PTSD words."""
     w = _words(ptsd)
     data = obj_text.encode() if isinstance(obj_text, str) else bytes(obj_text)
     mtl = None if mtl_text is None else (mtl_text.encode() if isinstance(mtl_text, str) else bytes(mtl_text))
     b = (ctypes.c_float * 3)(*[float(x) for x in base])
     p, n = _u32p(), ctypes.c_size_t(0)
-    _check(library().pt_ptsd_add_obj(w.ctypes.data, w.size, data, len(data), mtl, 0 if mtl is None else len(mtl), b,
-                                     float(height), int(material), float(turn_deg), ctypes.byref(p), ctypes.byref(n)))
+    if images is None:
+        _check(library().pt_ptsd_add_obj(w.ctypes.data, w.size, data, len(data), mtl, 0 if mtl is None else len(mtl),
+                                         b, float(height), int(material), float(turn_deg), ctypes.byref(p),
+                                         ctypes.byref(n)))
+    else:
+        keep = [(k.encode(), decode_image(v) if isinstance(v, (bytes, bytearray, memoryview))
+                 else np.ascontiguousarray(v, dtype=np.uint8)) for k, v in images.items()]
+        arr = (_Image * max(1, len(keep)))()
+        for i, (name, img) in enumerate(keep):
+            if img.ndim != 3 or img.shape[2] != 3:
+                raise ValueError(f"image {name.decode()!r}: expected a (height, width, 3) array of sRGB bytes")
+            arr[i] = _Image(name, img.shape[1], img.shape[0], img.ctypes.data)
+        _check(library().pt_ptsd_add_obj_textured(w.ctypes.data, w.size, data, len(data), mtl,
+                                                  0 if mtl is None else len(mtl), arr, len(keep), b, float(height),
+                                                  int(material), float(turn_deg), ctypes.byref(p), ctypes.byref(n)))
     return _take_words(p, n)
 
 

@@ -37,6 +37,7 @@ struct ptgp_check {
     section sec[8];
     uint32_t n_nodes, n_order, n_sph, n_tri, n_inst, n_mat, n_lights, n_large;
     uint32_t flat_root, inst_root, n_small;
+    uint32_t tex_off = 0, tex_bytes = 0, n_tex = 0;
     std::vector<float> leaf;   // 12 floats per order slot
     int depth_flat = 0, depth_inst = 0, depth_blas = 0;
 
@@ -70,9 +71,28 @@ struct ptgp_check {
         n_small = w[20];
         if (n_small > n_sph) bad("more small spheres than spheres");
         if (n_nodes >= (1u << 28) || n_order >= (1u << 28)) bad("more than 2^28 nodes or leaf slots");
+        // The texture section, this library's extension in words the
+        // reference writes as 0: header words 38-40 (offset, bytes, count).
+        tex_off = w[38];
+        tex_bytes = w[39];
+        n_tex = w[40];
+        if (tex_off || tex_bytes || n_tex) {
+            if (tex_off % 16 || tex_off < 192 || uint64_t(tex_off) + tex_bytes > 4 * uint64_t(n))
+                bad("texture section out of the buffer or not 16-byte aligned");
+            if (n_tex == 0 || tex_bytes % 16 || tex_bytes < 16ull * n_tex) bad("texture section size");
+            const uint64_t texels = (tex_bytes - 16ull * n_tex) / 16;
+            const uint32_t* info = w + tex_off / 4;
+            for (uint32_t t = 0; t < n_tex; ++t) {
+                const uint32_t tw = info[4 * t], th = info[4 * t + 1], first = info[4 * t + 2];
+                if (tw < 1 || th < 1 || tw > 65536 || th > 65536) bad("texture size out of range");
+                if (uint64_t(first) + uint64_t(tw) * th > texels) bad("texture texels out of the section");
+            }
+        }
         for (uint32_t m = 0; m < n_mat; ++m) {
             const float type = fbits(at(5)[12 * m + 3]);
             if (!(type == 0 || type == 1 || type == 2 || type == 3)) bad("material type not 0-3");
+            const uint32_t tex = at(5)[12 * m + 9];
+            if (tex && (type != 0 || tex > n_tex)) bad("material texture out of range or not on a matte material");
         }
         for (uint32_t s = 0; s < n_sph; ++s)
             if (at(2)[8 * s + 4] >= n_mat) bad("sphere material out of range");
@@ -350,6 +370,9 @@ pt_status pt_scene_create(const uint32_t* ptgp, size_t n_words, pt_scene** out) 
         v.n_cached = 0;
         v.cache = 0;
         v.n_top = std::min<uint32_t>(ptd::TOP_NODES, chk.n_nodes);
+        v.n_tex = chk.n_tex;
+        v.tex_info = chk.n_tex ? reinterpret_cast<const uint4*>(dw + chk.tex_off / 4) : nullptr;
+        v.texels = chk.n_tex ? reinterpret_cast<const float4*>(dw + chk.tex_off / 4) + chk.n_tex : nullptr;
         v.n_small = chk.n_small;
         v.flags = ptgp[21];
         v.n_lights = chk.n_lights;

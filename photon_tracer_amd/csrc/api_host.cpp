@@ -73,6 +73,52 @@ pt_status pt_ptsd_add_obj(const uint32_t* ptsd, size_t n_words, const char* obj_
     });
 }
 
+pt_status pt_ptsd_add_obj_textured(const uint32_t* ptsd, size_t n_words, const char* obj_text, size_t text_len,
+                                   const char* mtl_text, size_t mtl_len, const pt_image* images, size_t n_images,
+                                   const float base[3], double height, uint32_t material, double turn_deg,
+                                   uint32_t** out, size_t* out_words) {
+    if (!ptsd || !out || !out_words || !base || (text_len && !obj_text) || (mtl_len && !mtl_text) ||
+        (n_images && !images))
+        return pt::fail(PT_ERR_INVALID, "pt_ptsd_add_obj_textured: null pointer");
+    if (!(height > 0)) return pt::fail(PT_ERR_INVALID, "pt_ptsd_add_obj_textured: height must be > 0");
+    PT_GUARD({
+        std::vector<pt::image_rec> ims(n_images);
+        for (size_t k = 0; k < n_images; ++k) {
+            const pt_image& in = images[k];
+            if (!in.name || !in.rgb || in.width < 1 || in.height < 1 || in.width > 65536 || in.height > 65536)
+                throw pt::status_error(PT_ERR_INVALID, "pt_ptsd_add_obj_textured: image without a name, pixels "
+                                                       "or a size of 1 to 65536 a side");
+            ims[k].name = in.name;
+            ims[k].width = in.width;
+            ims[k].height = in.height;
+            ims[k].rgb.assign(in.rgb, in.rgb + size_t(in.width) * in.height * 3);
+        }
+        pt::scene_desc s = pt::ptsd_read(ptsd, n_words);
+        const std::string mtl(mtl_text ? mtl_text : "", mtl_len);
+        pt::add_obj(s, std::string(obj_text ? obj_text : "", text_len), pt::v3(base[0], base[1], base[2]), height,
+                    material, turn_deg, mtl_text ? &mtl : nullptr, &ims);
+        *out = pt::copy_out(pt::ptsd_write(s), out_words);
+    });
+}
+
+pt_status pt_decode_image(const void* bytes, size_t n_bytes, uint32_t* width, uint32_t* height, uint8_t** rgb) {
+    if (!width || !height || !rgb || (n_bytes && !bytes)) return pt::fail(PT_ERR_INVALID, "pt_decode_image: null pointer");
+    *rgb = nullptr;
+    PT_GUARD({
+        try {
+            const pt::image_rec img = pt::decode_image(static_cast<const uint8_t*>(bytes), n_bytes);
+            uint8_t* p = static_cast<uint8_t*>(std::malloc(img.rgb.size()));
+            if (!p) throw std::bad_alloc();
+            std::memcpy(p, img.rgb.data(), img.rgb.size());
+            *width = img.width;
+            *height = img.height;
+            *rgb = p;
+        } catch (const std::runtime_error& e) {
+            throw pt::status_error(PT_ERR_INVALID, std::string("pt_decode_image: ") + e.what());
+        }
+    });
+}
+
 pt_status pt_pack(const uint32_t* ptsd, size_t n_words, uint32_t** ptgp, size_t* ptgp_words) {
     if (!ptsd || !ptgp || !ptgp_words) return pt::fail(PT_ERR_INVALID, "pt_pack: null pointer");
     PT_GUARD({

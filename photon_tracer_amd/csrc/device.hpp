@@ -44,7 +44,10 @@ struct SceneView {
     // nodes every ray visits are the cached ones.
     uint32_t n_cached, cache;
     uint32_t n_top;          // nodes the host numbered first (at most TOP_NODES), available for the cache
-    uint32_t pad[3];
+    uint32_t n_tex;          // textures (PTGP texture section; 0: none)
+    uint32_t pad[2];
+    const uint4* tex_info;   // per texture: width, height, first texel, 0
+    const float4* texels;    // linear RGB (a = 0), rows top first
     float4 cam[4];           // origin, corner, across, up
 };
 constexpr uint32_t NODE_CACHE_STRIDE = 144;   // bytes per cached node: 7 rows of 16, padded against bank conflicts
@@ -162,6 +165,30 @@ __device__ __forceinline__ bool tri_hit(f3 p0, f3 e1, f3 e2, f3 o, f3 d, float t
     u_out = u;
     v_out = v;
     return true;
+}
+
+// image_texture::sample (texture.hpp:22-33): bilinear, repeating outside
+// [0, 1), v running up the image whose rows are stored top first; the
+// reference's float operations in its order (no FMA contraction).
+__device__ __forceinline__ f3 texture_sample(const SceneView& sc, uint32_t t, float u, float v) {
+    const uint4 info = sc.tex_info[t];
+    const int w = static_cast<int>(info.x), h = static_cast<int>(info.y);
+    const float4* tx = sc.texels + info.z;
+    const float x = (u - floorf(u)) * static_cast<float>(w) - 0.5f;
+    const float y = (1 - (v - floorf(v))) * static_cast<float>(h) - 0.5f;
+    // Finite coordinates give x in [-0.5, w - 0.5], y in [-0.5, h - 0.5]; a
+    // NaN or infinite one reads texel 0 (its weights are NaN, as on the CPU).
+    auto cell = [](float c) { const float f = floorf(c); return f >= -1.0f && f <= 65536.0f ? static_cast<int>(f) : 0; };
+    const int x0 = cell(x), y0 = cell(y);
+    const float fx = x - static_cast<float>(x0), fy = y - static_cast<float>(y0);
+    auto texel = [&](int xi, int yi) {
+        xi = ((xi % w) + w) % w;
+        yi = ((yi % h) + h) % h;
+        const float4 c = tx[static_cast<size_t>(yi) * w + xi];
+        return f3{c.x, c.y, c.z};
+    };
+    return texel(x0, y0) * ((1 - fx) * (1 - fy)) + texel(x0 + 1, y0) * (fx * (1 - fy)) +
+           texel(x0, y0 + 1) * ((1 - fx) * fy) + texel(x0 + 1, y0 + 1) * (fx * fy);
 }
 
 // ---------------------------------------------------------------------------

@@ -68,13 +68,16 @@ struct box3 {
 //   word 0 magic 'PTSD' (0x44535450), 1 version (1), 2 flags (1 sky, 2 NEE),
 //   3-5 lookfrom, 6-8 lookat, 9 vfov (degrees, float), 10 n_materials,
 //   11 n_spheres, 12 n_large, 13 n_triangles, 14 n_meshes, 15 n_instances,
-//   16 mesh triangles in total, 17-31 reserved (0); then the arrays:
-//   materials  12 words: albedo(3) type emission(3) fuzz ior 0 0 0
+//   16 mesh triangles in total, 17 n_textures, 18-31 reserved (0); then the arrays:
+//   materials  12 words: albedo(3) type emission(3) fuzz ior texture 0 0
+//              (texture: 0 none, else texture number + 1; matte only)
 //   spheres     5 words: center(3) radius material
 //   large       5 words: as spheres
 //   triangles  28 words: p0 p1 p2 n0 n1 n2 (18) uv(6) material 0 0 0
 //   mesh sizes  n_meshes words, then each mesh's triangles (28 words each)
 //   instances  16 words: mesh material(~0u: none) m[3][3] (row major) t(3) 0 0
+//   textures   width height, then width x height x 3 bytes of sRGB (rows top
+//              first, as image8 holds them, image_io.hpp), padded to a word
 // Material types follow gpu_pack.hpp:93-106: 0 lambertian, 1 metal,
 // 2 dielectric, 3 light.
 // ---------------------------------------------------------------------------
@@ -88,6 +91,7 @@ struct material_rec {
     uint32_t type = 0;
     v3 emission;
     float fuzz = 0, ior = 1.5f;
+    uint32_t texture = 0;   // matte: 0 none, else texture number + 1 (albedo x texture, material.hpp:72)
     // Only lights carry emission (material.hpp:97-102); material::emits (material.hpp:29).
     bool is_light() const { return type == 3 && (emission.x > 0 || emission.y > 0 || emission.z > 0); }
 };
@@ -120,6 +124,18 @@ struct instance_rec {
     affine_rec to_world;
 };
 
+// An 8-bit sRGB image (image8, image_io.hpp): rows top first, 3 bytes a
+// texel. `name` is only used to match OBJ/MTL map_Kd names.
+struct image_rec {
+    std::string name;
+    uint32_t width = 0, height = 0;
+    std::vector<uint8_t> rgb;
+};
+
+// decode_image (image_io.hpp:295): a PNG or PPM file's bytes, told apart by
+// their first bytes, as 8-bit sRGB (image.cpp). Throws on what it cannot read.
+image_rec decode_image(const uint8_t* bytes, size_t size);
+
 struct scene_desc {
     bool sky = true, sample_lights = true;
     v3 lookfrom{0, 0, 0}, lookat{0, 0, -1};
@@ -129,6 +145,7 @@ struct scene_desc {
     std::vector<triangle_rec> triangles;
     std::vector<std::vector<triangle_rec>> meshes;
     std::vector<instance_rec> instances;
+    std::vector<image_rec> textures;
 
     uint32_t add_material(const material_rec& m) {
         materials.push_back(m);
@@ -177,9 +194,11 @@ std::vector<uint32_t> pack_ptgp(const scene_desc& s, const bvh_builder* builder 
 
 // load_obj (obj.hpp:114-218): the OBJ text's faces appended to s.triangles,
 // with materials from the MTL text (standing in for every library the OBJ
-// names; appended to s.materials) or `material` (obj.cpp).
+// names; appended to s.materials) or `material` (obj.cpp). A matte
+// material's map_Kd that names one of `images` gets that image as a texture
+// (appended to s.textures).
 void add_obj(scene_desc& s, const std::string& text, v3 base, double height, uint32_t material, double turn_deg,
-             const std::string* mtl_text = nullptr);
+             const std::string* mtl_text = nullptr, const std::vector<image_rec>* images = nullptr);
 
 // A malloc'd copy of v for the C ABI (free with pt_free); *n = v.size().
 uint32_t* copy_out(const std::vector<uint32_t>& v, size_t* n);

@@ -62,6 +62,7 @@ std::vector<uint32_t> merge_ptgp(const std::vector<ptgp_in>& in, std::vector<mer
         S[k] = sections_of(in[k]);
         const ptgp_sections& s = S[k];
         if (s.inst_root != ~0u || s.count[4] != 0) throw std::runtime_error("batched scenes must have no instances");
+        if (s.w[38] || s.w[39] || s.w[40]) throw std::runtime_error("batched scenes must have no textures");
         if (s.n_small > s.count[2]) throw std::runtime_error("more small spheres than spheres");
         n_nodes += s.count[0];
         n_order += s.count[1];

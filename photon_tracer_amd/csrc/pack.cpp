@@ -92,13 +92,14 @@ std::vector<uint32_t> ptsd_write(const scene_desc& s) {
     size_t mesh_tris = 0;
     for (const auto& m : s.meshes) mesh_tris += m.size();
     o[16] = static_cast<uint32_t>(mesh_tris);
+    o[17] = static_cast<uint32_t>(s.textures.size());
     for (const auto& m : s.materials) {
         put(o, m.albedo);
         o.push_back(m.type);
         put(o, m.emission);
         put(o, m.fuzz);
         put(o, m.ior);
-        o.push_back(0);
+        o.push_back(m.texture);
         o.push_back(0);
         o.push_back(0);
     }
@@ -121,6 +122,13 @@ std::vector<uint32_t> ptsd_write(const scene_desc& s) {
         o.push_back(0);
         o.push_back(0);
     }
+    for (const auto& im : s.textures) {
+        o.push_back(im.width);
+        o.push_back(im.height);
+        const size_t at = o.size();
+        o.resize(at + (im.rgb.size() + 3) / 4, 0);
+        if (!im.rgb.empty()) std::memcpy(&o[at], im.rgb.data(), im.rgb.size());
+    }
     return o;
 }
 
@@ -135,8 +143,18 @@ scene_desc ptsd_read(const uint32_t* w, size_t n) {
     s.lookat = {fbits(w[6]), fbits(w[7]), fbits(w[8])};
     s.vfov = fbits(w[9]);
     const uint32_t n_mat = w[10], n_sph = w[11], n_large = w[12], n_tri = w[13], n_mesh = w[14], n_inst = w[15];
-    const size_t need = kPtsdHeader + size_t(n_mat) * kMatWords + size_t(n_sph + n_large) * kSphWords +
-                        size_t(n_tri) * kTriWords + n_mesh + size_t(w[16]) * kTriWords + size_t(n_inst) * kInstWords;
+    const uint32_t n_tex = w[17];
+    const size_t fixed = kPtsdHeader + size_t(n_mat) * kMatWords + size_t(n_sph + n_large) * kSphWords +
+                         size_t(n_tri) * kTriWords + n_mesh + size_t(w[16]) * kTriWords + size_t(n_inst) * kInstWords;
+    // The textures' sizes come from their own headers.
+    size_t need = fixed;
+    for (uint32_t k = 0; k < n_tex; ++k) {
+        if (need + 2 > n) throw std::runtime_error("PTSD buffer size does not match its header");
+        if (w[need] == 0 || w[need + 1] == 0 || w[need] > 65536 || w[need + 1] > 65536)
+            throw std::runtime_error("PTSD texture size out of range (1 to 65536 a side)");
+        const uint64_t texels = uint64_t(w[need]) * w[need + 1];
+        need += 2 + size_t((texels * 3 + 3) / 4);
+    }
     if (need != n) throw std::runtime_error("PTSD buffer size does not match its header");
     reader r{w, n, kPtsdHeader};
     s.materials.resize(n_mat);
@@ -146,10 +164,12 @@ scene_desc ptsd_read(const uint32_t* w, size_t n) {
         m.emission = r.vec();
         m.fuzz = r.f();
         m.ior = r.f();
-        r.u();
+        m.texture = r.u();
         r.u();
         r.u();
         if (m.type > 3) throw std::runtime_error("PTSD material type out of range");
+        if (m.texture > n_tex || (m.texture && m.type != 0))
+            throw std::runtime_error("PTSD material texture out of range or not on a matte material");
     }
     auto read_sphere = [&](sphere_rec& sp) {
         sp.center = r.vec();
@@ -193,6 +213,14 @@ scene_desc ptsd_read(const uint32_t* w, size_t n) {
         if (in.mesh >= n_mesh) throw std::runtime_error("PTSD instance mesh out of range");
         if (in.material != ~0u && in.material >= n_mat)
             throw std::runtime_error("PTSD instance material out of range");
+    }
+    s.textures.resize(n_tex);
+    for (auto& im : s.textures) {
+        im.width = r.u();
+        im.height = r.u();
+        im.rgb.resize(size_t(im.width) * im.height * 3);
+        std::memcpy(im.rgb.data(), w + r.at, im.rgb.size());
+        r.at += (im.rgb.size() + 3) / 4;
     }
     return s;
 }
@@ -345,7 +373,10 @@ std::vector<uint32_t> pack_ptgp(const scene_desc& s, const bvh_builder* builder)
             type = 3;
         }
         const v3 e = m.type == 3 ? m.emission : v3(0, 0, 0);
-        const float row[12] = {albedo.x, albedo.y, albedo.z, type, e.x, e.y, e.z, fuzz, ior, 0, 0, 0};
+        // Word 9: a matte material's texture (number + 1, 0 none): this
+        // library's extension of PTGP v1, where the reference writes 0.
+        const float tex = fbits(m.type == 0 ? m.texture : 0u);
+        const float row[12] = {albedo.x, albedo.y, albedo.z, type, e.x, e.y, e.z, fuzz, ior, tex, 0, 0};
         materials.insert(materials.end(), row, row + 12);
         const uint32_t k = static_cast<uint32_t>(index.size());
         index.emplace(id, k);
@@ -431,6 +462,39 @@ std::vector<uint32_t> pack_ptgp(const scene_desc& s, const bvh_builder* builder)
     out[20] = small_spheres;
     out[21] = (s.sky ? 1u : 0u) | (s.sample_lights ? 2u : 0u);
     std::memcpy(&out[22], camera, sizeof camera);
+    if (!s.textures.empty()) {
+        // The texture section (header words 38-40: byte offset, length and
+        // number of textures; 0 in the reference's PTGP, which drops
+        // textures, gpu_pack.hpp:96):
+        // per texture (width, height, first texel, 0), then every texture's
+        // texels as linear RGB float4 (a = 0), rows top first. The sRGB bytes
+        // are linearised as image_texture's table does (texture.hpp:13-20).
+        float linear[256];
+        for (int v = 0; v < 256; ++v) {
+            const float c = v / 255.0f;
+            linear[v] = c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f);
+        }
+        std::vector<uint32_t> sec;
+        uint64_t first = 0;
+        for (const auto& im : s.textures) {
+            sec.insert(sec.end(), {im.width, im.height, static_cast<uint32_t>(first), 0u});
+            first += uint64_t(im.width) * im.height;
+            if (first >= (uint64_t(1) << 32)) throw std::runtime_error("textures too large for PTGP");
+        }
+        sec.reserve(sec.size() + 4 * first);
+        for (const auto& im : s.textures)
+            for (size_t k = 0; k < size_t(im.width) * im.height; ++k) {
+                for (int c = 0; c < 3; ++c) sec.push_back(ubits(linear[im.rgb[3 * k + c]]));
+                sec.push_back(0);
+            }
+        while (out.size() % 4) out.push_back(0);   // float4 texels: the section starts 16-byte aligned
+        const size_t at = out.size();
+        if (4 * (at + sec.size()) > 0xffffffffull) throw std::runtime_error("scene too large for the PTGP 32-bit offsets");
+        out[38] = static_cast<uint32_t>(4 * at);
+        out[39] = static_cast<uint32_t>(4 * sec.size());
+        out[40] = static_cast<uint32_t>(s.textures.size());
+        out.insert(out.end(), sec.begin(), sec.end());
+    }
     return out;
 }
 

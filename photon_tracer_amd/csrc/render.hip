@@ -426,6 +426,7 @@ struct Surface {
     f3 pnt, nrm;
     bool front, sampled_light;
     uint32_t mat;
+    float tu, tv;   // texture coordinates (triangle.hpp:72-73; 0 on spheres, hittable.hpp:14)
 };
 
 __device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& h, f3 o, f3 d) {
@@ -437,6 +438,7 @@ __device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& 
     bool front;
     uint32_t mat;
     bool sampled_light = false;
+    sf.tu = sf.tv = 0;
     if (h.kind <= 2) {
         const float4 cr = sc.spheres[2 * h.prim];
         mat = __float_as_uint(sc.spheres[2 * h.prim + 1].x);
@@ -453,6 +455,9 @@ __device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& 
         const f3 n0 = xyz(tr[3]), n1 = xyz(tr[4]), n2 = xyz(tr[5]), fn = xyz(tr[6]);
         const float b0 = 1 - h.u - h.v;
         const f3 sn = normalize(n0 * b0 + n1 * h.u + n2 * h.v);
+        // uv in the rows' w words: (e1, u0) (e2, v0) (n0, u1) (n1, v1) (n2, u2) (fn, v2)
+        sf.tu = tr[1].w * b0 + tr[3].w * h.u + tr[5].w * h.v;
+        sf.tv = tr[2].w * b0 + tr[4].w * h.u + tr[6].w * h.v;
         if (h.inst == NONE) {
             front = dot(d, fn) < 0;
             nrm = front ? sn : -sn;
@@ -530,6 +535,9 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
     // material::scatter (material.hpp:59-137)
     const int type = static_cast<int>(m0.w);
     f3 att = xyz(m0), dir = mk(0, 0, 0);
+    // A textured matte's albedo times its texture (material.hpp:72).
+    const uint32_t tex = __float_as_uint(m2.y);
+    if (type == 0 && tex != 0) att = mul(att, texture_sample(sc, tex - 1, sf.tu, sf.tv));
     bool ok = true;
     f3 wgt = att;          // the scattered ray's throughput factor (att unless guided)
     bool go_on = true;

@@ -14,8 +14,13 @@
                       unchanged: the OBJ loader's input.
   spot_scene.npz,     the reference's mesh scene loading Keenan Crane's Spot
   spot.obj.gz,        through its file reader (models/spot: the OBJ names
-  spot.mtl            spot.mtl, a textured matte), exported as PTSD, and the
-                      two input files copied unchanged: the MTL path's fixture.
+  spot.mtl,           spot.mtl, a matte textured by spot_texture.png),
+  spot_texture.png    exported as PTSD with the texture as the reference's
+                      decoder read it (spot_ptsd), the reference's "spot
+                      crowd" render-test scene (tests/render_test.py:55:
+                      crowd of 6 x 6, turned 150 degrees) exported the same
+                      way (spot_crowd_ptsd), and the three input files copied
+                      unchanged: the MTL, texture and image decoder fixture.
   mtl_kinds.npz       a small OBJ and MTL of our own (one quad per material:
                       a light, glass by dissolve, by Tr and by illum 7, metals
                       from illum 3 and 5 with and without Ks, a gray Kd from
@@ -63,12 +68,15 @@ def main():
 
 
 def write_spot():
-    ptsd, _ = ob.ref_scene("mesh", obj_path=f"{REF}/models/spot/spot.obj")
-    np.savez_compressed(os.path.join(HERE, "spot_scene.npz"), spot_ptsd=ptsd)
-    with open(f"{REF}/models/spot/spot.obj", "rb") as src, gzip.open(os.path.join(HERE, "spot.obj.gz"), "wb") as dst:
+    obj = f"{REF}/models/spot/spot.obj"
+    ptsd, _ = ob.ref_scene("mesh", obj_path=obj)
+    crowd, _ = ob.ref_scene("crowd", crowd_side=6, obj_path=obj, turn_deg=150)
+    np.savez_compressed(os.path.join(HERE, "spot_scene.npz"), spot_ptsd=ptsd, spot_crowd_ptsd=crowd)
+    with open(obj, "rb") as src, gzip.open(os.path.join(HERE, "spot.obj.gz"), "wb") as dst:
         shutil.copyfileobj(src, dst)
     shutil.copyfile(f"{REF}/models/spot/spot.mtl", os.path.join(HERE, "spot.mtl"))
-    print("spot", ptsd.size)
+    shutil.copyfile(f"{REF}/models/spot/spot_texture.png", os.path.join(HERE, "spot_texture.png"))
+    print("spot", ptsd.size, "spot crowd", crowd.size)
 
 
 MTL_KINDS = """newmtl lamp

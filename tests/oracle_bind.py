@@ -37,6 +37,8 @@ def oracle():
         lib.or_render.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int]
         lib.or_render.restype = ctypes.c_uint64
+        lib.or_texture_sample.argtypes = [_vp, ctypes.c_uint32, _vp, _sz, _vp]
+        lib.or_texture_sample.restype = None
         _oracle = lib
     return _oracle
 
@@ -59,8 +61,8 @@ def ref():
                                        ctypes.POINTER(_u32p), ctypes.POINTER(_sz)]
         lib.ref_world_camera.argtypes = [_vp, _vp]
         lib.ref_world_camera.restype = None
-        lib.ref_scene.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(_u32p),
-                                  ctypes.POINTER(_sz), ctypes.POINTER(_u32p), ctypes.POINTER(_sz)]
+        lib.ref_scene.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_double,
+                                  ctypes.POINTER(_u32p), ctypes.POINTER(_sz), ctypes.POINTER(_u32p), ctypes.POINTER(_sz)]
         lib.ref_world_trace.argtypes = [_vp, _vp, _sz, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
         lib.ref_world_trace.restype = ctypes.c_long
         lib.ref_world_render.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
@@ -69,6 +71,10 @@ def ref():
         lib.ref_sample_2d.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         lib.ref_hardware_threads.restype = ctypes.c_int
+        lib.ref_texture_sample.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _sz, _vp]
+        lib.ref_texture_sample.restype = None
+        lib.ref_decode_image.argtypes = [ctypes.c_char_p, _sz, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))]
         _ref = lib
     return _ref
 
@@ -131,11 +137,11 @@ def or_render(ptgp, width, height, spp, depth=10, seed=0, first_sample=0, nee=Tr
 REF_SCENES = {"materials": 0, "field": 1, "mesh": 2, "room": 3, "crowd": 4}
 
 
-def ref_scene(name: str, crowd_side: int = 8, flatten: bool = False, obj_path: str = ""):
+def ref_scene(name: str, crowd_side: int = 8, flatten: bool = False, obj_path: str = "", turn_deg: float = 0.0):
     """(PTSD, PTGP) of one of the reference's own scenes, built and packed by the reference."""
     lib = ref()
     a, na, b, nb = _u32p(), _sz(0), _u32p(), _sz(0)
-    if lib.ref_scene(REF_SCENES[name], crowd_side, 1 if flatten else 0, obj_path.encode(), ctypes.byref(a),
+    if lib.ref_scene(REF_SCENES[name], crowd_side, 1 if flatten else 0, obj_path.encode(), float(turn_deg), ctypes.byref(a),
                      ctypes.byref(na), ctypes.byref(b), ctypes.byref(nb)):
         raise RuntimeError(lib.ref_last_error().decode())
     return _take(lib, a, na), _take(lib, b, nb)
@@ -216,6 +222,37 @@ def ref_denoise(rgb, aux, spp, threads: int = THREADS):
                                 ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     lib.ref_denoise(w, h, int(spp), rgb.ctypes.data, aux.ctypes.data, out.ctypes.data, threads, ctypes.byref(secs))
     return out, secs.value
+
+
+def or_texture_sample(ptgp, t: int, uv) -> np.ndarray:
+    """The oracle's image_texture::sample of texture t of a PTGP buffer at (n, 2) uv: (n, 3) floats."""
+    p = np.ascontiguousarray(ptgp, np.uint32)
+    q = np.ascontiguousarray(uv, np.float32)
+    out = np.zeros((len(q), 3), np.float32)
+    oracle().or_texture_sample(p.ctypes.data, t, q.ctypes.data, len(q), out.ctypes.data)
+    return out
+
+
+def ref_texture_sample(rgb, uv) -> np.ndarray:
+    """The reference's image_texture::sample over (h, w, 3) sRGB bytes at (n, 2) uv: (n, 3) floats."""
+    img = np.ascontiguousarray(rgb, np.uint8)
+    q = np.ascontiguousarray(uv, np.float32)
+    out = np.zeros((len(q), 3), np.float32)
+    ref().ref_texture_sample(img.ctypes.data, img.shape[1], img.shape[0], q.ctypes.data, len(q), out.ctypes.data)
+    return out
+
+
+def ref_decode_image(data: bytes) -> np.ndarray:
+    """The reference's decode_image of a file's bytes: (h, w, 3) uint8; RuntimeError with its message."""
+    lib = ref()
+    w, h, p = ctypes.c_int(), ctypes.c_int(), ctypes.POINTER(ctypes.c_uint8)()
+    if lib.ref_decode_image(data, len(data), ctypes.byref(w), ctypes.byref(h), ctypes.byref(p)) != 0:
+        raise RuntimeError(lib.ref_last_error().decode())
+    try:
+        n = 3 * w.value * h.value
+        return np.ctypeslib.as_array(p, (max(n, 1),))[:n].copy().reshape(h.value, w.value, 3)
+    finally:
+        lib.ref_free(p)
 
 
 def ref_sample_2d(seed, i, j, index, pair):

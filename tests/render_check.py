@@ -28,6 +28,30 @@ def load_scene(name: str):
     return z[f"{name}_ptsd"], z[f"{name}_ptgp"]
 
 
+def load_spot(name: str = "spot_crowd"):
+    """A PTSD scene of the reference's textured Spot, as the reference built it
+    and exported it with its decoded texture: "spot" (its mesh scene loading
+    models/spot/spot.obj) or "spot_crowd" (tests/render_test.py:55: a crowd
+    of 6 x 6 turned 150 degrees)."""
+    return np.load(os.path.join(GOLDEN, "spot_scene.npz"))[f"{name}_ptsd"]
+
+
+def ptsd_textures(w: np.ndarray) -> list:
+    """The textures of a PTSD buffer as (h, w, 3) uint8 sRGB arrays: after the
+    header, materials, spheres, triangles, mesh sizes and triangles and the
+    instances, each texture's width and height then its bytes, word padded."""
+    pos = 32 + 12 * int(w[10]) + 5 * (int(w[11]) + int(w[12])) + 28 * int(w[13])
+    pos += int(w[14]) + 28 * int(w[16]) + 16 * int(w[15])
+    out = []
+    for _ in range(int(w[17])):
+        tw, th = int(w[pos]), int(w[pos + 1])
+        nb = 3 * tw * th
+        out.append(w[pos + 2:pos + 2 + (nb + 3) // 4].view(np.uint8)[:nb].reshape(th, tw, 3))
+        pos += 2 + (nb + 3) // 4
+    assert pos == len(w)
+    return out
+
+
 def block_means(img: np.ndarray, block: int = BLOCK) -> np.ndarray:
     """(bh, bw, 3) block means of an (h, w, 3) image, top row first."""
     h, w, _ = img.shape

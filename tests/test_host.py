@@ -236,23 +236,33 @@ f 1 2 5
 
 def test_obj_loader_with_mtl_matches_reference_load_obj():
     # The reference's mesh scene loading Spot through its file reader: the OBJ
-    # names spot.mtl (a matte with a texture map). Our loader, given the same
-    # MTL text, must add the same triangles and a material with the same
-    # words; the texture is not carried by PTSD/PTGP on either side.
+    # names spot.mtl, a matte textured by spot_texture.png. Our loader, given
+    # the same MTL text and the image the reference decoded, must add the same
+    # triangles, a material with the same words and the same texture.
     import gzip
-    ref_ptsd = np.load(os.path.join(GOLDEN, "spot_scene.npz"))["spot_ptsd"]
+    ref_ptsd = rc.load_spot("spot")
     text = gzip.open(os.path.join(GOLDEN, "spot.obj.gz")).read()
     mtl = open(os.path.join(GOLDEN, "spot.mtl"), "rb").read()
     ref = ptsd_triangles(ref_ptsd)
     n_mat = int(ref_ptsd[10])
-    out = pt.add_obj(ref_ptsd, text, (0.0, -0.5, -1.2), 0.75, 0, mtl_text=mtl)
+    (image,) = rc.ptsd_textures(ref_ptsd)
+    out = pt.add_obj(ref_ptsd, text, (0.0, -0.5, -1.2), 0.75, 0, mtl_text=mtl, images={"spot_texture.png": image})
     assert int(out[10]) == n_mat + 1                       # one material made, appended
     tris = ptsd_triangles(out)[len(ref):]
     assert len(tris) == len(ref) and (tris[:, 24] == n_mat).all()
     assert np.array_equal(np.delete(tris, 24, axis=1), np.delete(ref, 24, axis=1))
     mats = out[32:32 + 12 * (n_mat + 1)].reshape(-1, 12)
-    assert np.array_equal(mats[n_mat], mats[int(ref[0, 24])])
+    theirs = mats[int(ref[0, 24])]
     assert mats[n_mat, 3] == 0 and np.array_equal(mats[n_mat, :3].view(np.float32), [1, 1, 1])   # matte, Kd 1 1 1
+    assert np.array_equal(np.delete(mats[n_mat], 9), np.delete(theirs, 9))
+    # word 9: texture number + 1; ours is the scene's second texture, holding the same image
+    assert theirs[9] == 1 and mats[n_mat, 9] == 2 and int(out[17]) == 2
+    textures = rc.ptsd_textures(out)
+    assert np.array_equal(textures[1], image) and np.array_equal(textures[0], image)
+    # Without the image the material is an untextured matte, as the reference makes it when its
+    # reader finds no file (obj.hpp:100-104).
+    plain_tex = pt.add_obj(ref_ptsd, text, (0.0, -0.5, -1.2), 0.75, 0, mtl_text=mtl)
+    assert plain_tex[32 + 12 * n_mat + 9] == 0 and int(plain_tex[17]) == 1
     # Without the MTL text every face takes the given material, as the browser demo loads.
     plain = ptsd_triangles(pt.add_obj(ref_ptsd, text, (0.0, -0.5, -1.2), 0.75, 2))[len(ref):]
     assert (plain[:, 24] == 2).all()
