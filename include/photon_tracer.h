@@ -160,7 +160,9 @@ pt_status pt_pack_ex(const uint32_t* ptsd, size_t n_words, int flags, uint32_t**
  * build, the device's transfers and allocations included, the copies into
  * the returned arrays not (may be NULL). At most 2^30 boxes (2^28 for the
  * device build, PT_ERR_LIMIT); the device build refuses boxes with a NaN
- * coordinate (PT_ERR_INVALID). */
+ * coordinate (PT_ERR_INVALID) and hands boxes with a -0.0 coordinate to the
+ * host builder (the reference keeps whichever of -0.0 and +0.0 comes first;
+ * the device's reductions cannot), so the words are the same either way. */
 enum { PT_BVH_DEVICE = 1 };
 pt_status pt_bvh_build(const float* boxes, size_t n, int flags, uint32_t** nodes, size_t* n_nodes, uint32_t** order,
                        double* ms);

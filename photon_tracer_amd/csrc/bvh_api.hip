@@ -43,10 +43,19 @@ pt::bvh_tree device_bvh(const std::vector<pt::box3>& boxes) {
         throw pt::status_error(PT_ERR_LIMIT, "device BVH build: more than 2^28 primitives");
     // With a NaN, std::min's result depends on the order it is taken in, so
     // the parallel build could not promise the host's tree: refused.
+    bool negative_zero = false;
     for (const pt::box3& b : boxes)
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a) {
             if (std::isnan(b.lo[a]) || std::isnan(b.hi[a]))
                 throw pt::status_error(PT_ERR_INVALID, "device BVH build: a box has a NaN coordinate");
+            negative_zero |= (b.lo[a] == 0 && std::signbit(b.lo[a])) || (b.hi[a] == 0 && std::signbit(b.hi[a]));
+        }
+    // -0 and +0 compare equal, so std::min and std::max keep whichever of the
+    // two comes first in the reference's order, while the device reductions
+    // (atomics on order-preserving keys, shuffles) always pick one sign. The
+    // node words could then differ in a sign bit: such boxes are built by the
+    // host builder, which takes the reference's order.
+    if (negative_zero) return pt::build_bvh(boxes);
     Pool& P = pool();
     std::lock_guard<std::mutex> lock(P.m);
     P.boxes.ensure(24 * n);

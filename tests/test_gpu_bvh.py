@@ -40,6 +40,14 @@ def make_boxes(kind, n, seed=1):
     elif kind == "grid":
         c = np.stack([np.floor(rng.uniform(0, 4, n)), np.floor(rng.uniform(0, 3, n)), np.full(n, 0.5)], 1)
         h = np.full((n, 3), 0.1)
+    elif kind == "signed zeros":
+        # flat boxes on the plane z = 0, their z bounds -0.0 or +0.0 by turns (OBJ files write "-0.000000"):
+        # the reference's std::min/std::max keep whichever comes first
+        c = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), np.zeros(n)], 1)
+        h = np.stack([np.full(n, 0.01), np.full(n, 0.01), np.zeros(n)], 1)
+        b = np.concatenate([c - h, c + h], 1).astype(np.float32)
+        b[::2, 2], b[1::2, 2], b[::3, 5] = -0.0, 0.0, -0.0
+        return b
     elif kind == "mixed":
         c = 100 * rng.normal(size=(n, 3))
         r = np.where(rng.uniform(size=n) < 0.01, 50 * rng.uniform(size=n), 0.2 * rng.uniform(size=n))
@@ -59,7 +67,7 @@ def leaf_primitives(nodes, order):
 
 CASES = [("uniform", n) for n in (1, 2, 3, 8, 9, 64, 500, 1024, 1025, 5000, 100_000)] + [
     ("clusters", 12_000), ("same", 8), ("same", 9), ("same", 3000), ("geometric", 300), ("geometric", 3000),
-    ("planar", 9000), ("grid", 6000), ("mixed", 20_000), ("uniform", 1_000_000)]
+    ("planar", 9000), ("grid", 6000), ("mixed", 20_000), ("signed zeros", 5000), ("uniform", 1_000_000)]
 
 
 @pytest.mark.parametrize("kind,n", CASES)
@@ -71,6 +79,8 @@ def test_device_build_equals_host_build(kind, n):
     assert dn.shape == hn.shape
     assert np.array_equal(do, ho)
     assert np.array_equal(dn, hn)
+    if kind == "signed zeros":   # both zeros reach the node words
+        assert {0x80000000, 0} <= set(dn[:, 8:12].reshape(-1).tolist()) | set(dn[:, 20:24].reshape(-1).tolist())
     # and a sanity check that does not lean on the host builder: each primitive in exactly one leaf
     assert np.array_equal(np.sort(leaf_primitives(dn, do)), np.arange(n, dtype=np.uint32))
 
