@@ -46,6 +46,7 @@ python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -222,7 +223,7 @@ def cpu_baseline(ptsd, width, height, spp_full, seconds, repeats=3):
     }
 
 
-def batch_leg(pt, n_jobs, steps, tune=True, graph=True):
+def batch_leg(pt, n_jobs, steps, tune=True, graph=True, compare=True):
     """The batched workload: n_jobs seeded jobs, generated and packed on host
     threads, merged into one device scene, tuned, then rendered in one call
     per step; the same jobs one pt_render call at a time for comparison."""
@@ -262,12 +263,25 @@ def batch_leg(pt, n_jobs, steps, tune=True, graph=True):
             "ms_per_batch": round(ms / steps, 2),
             "wall_mrays_per_s": round(rays / sum(walls) / 1e6, 1),
         }
+        out["mode"] = "shared-queue passes: slices of every job in one bounce loop per pass"
+        out["passes"] = st.batches
         if tuned:
             out["tuned"] = {"stack_entries": tuned.stack_entries, "blocks_per_cu": tuned.blocks_per_cu,
-                            "streams": tuned.streams, "max_paths": tuned.max_paths}
-            out["tuner_tried"] = [{"stack_entries": t[0], "blocks_per_cu": t[1], "streams": t[2], "max_paths": t[3],
+                            "max_paths": tuned.max_paths}
+            out["tuner_tried"] = [{"stack_entries": t[0], "blocks_per_cu": t[1], "max_paths": t[3],
                                    "mrays_per_s": round(t[4], 1)} for t in tuned.tried]
+        if compare:
+            # The same jobs as per-job bounce loops over 4 HIP streams (PT_BATCH_STREAMS), for comparison.
+            b.render(bjobs, device_out=out_buf, graph=graph, shared=False, streams=4)
+            srays, sms = 0, 0.0
+            for k in range(steps):
+                _, sst = b.render(bjobs, device_out=out_buf, graph=graph, shared=False, streams=4)
+                srays += sst.rays
+                sms += sst.render_ms
+            out["per_job_streams_mrays_per_s"] = round(srays / (sms * 1e-3) / 1e6, 1)
         out_buf.free()
+    if not compare:
+        return out
     # The same jobs one pt_render call at a time, each its own scene, resident
     # and warmed up; device time summed and wall time of the calls.
     scenes = [pt.Scene(g) for g in ptgps]
@@ -286,6 +300,7 @@ def batch_leg(pt, n_jobs, steps, tune=True, graph=True):
         s.close()
     out["one_call_per_job_mrays_per_s"] = round(rays / (ms * 1e-3) / 1e6, 1)
     out["one_call_per_job_wall_mrays_per_s"] = round(rays / wall / 1e6, 1)
+    out["batched_over_one_call_per_job"] = round(out["mrays_per_s"] / out["one_call_per_job_mrays_per_s"], 3)
     log(f"batch: {out['mrays_per_s']} Mrays/s batched ({out['wall_mrays_per_s']} wall), "
         f"{out['one_call_per_job_mrays_per_s']} one call per job ({out['one_call_per_job_wall_mrays_per_s']} wall)")
     return out
@@ -371,16 +386,31 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
             if not runs:
                 net = network_roofline(s, W, H, 16)
                 log(f"guided: network roofline {json.dumps(net)}")
+            # The time to reach the plain tracer's relMSE, measured: a guided
+            # render (new seed) at the sample count where relMSE ~ 1/spp
+            # predicts the match, in whole 16-spp batches, and 25% more
+            # samples at a time (at most three times) until its relMSE is at
+            # most plain's. Training time counts.
+            t_pred = train_ms + sg.render_ms * e_guided / e_plain
+            spp_m = max(16, math.ceil(spp_g * e_guided / e_plain / 16) * 16)
+            for attempt in range(4):
+                gm, _, sm = s.render(W, H, spp_m, seed=61 + attempt, guide=True, samples_per_batch=16)
+                e_m = rel_mse(gm)
+                if e_m <= e_plain or attempt == 3:
+                    break
+                spp_m = math.ceil(spp_m * 1.25 / 16) * 16
             s.set_guide(None)
             g.close()
-            # relMSE falls as 1/spp: the guided time to reach the plain tracer's relMSE.
-            t_match = train_ms + sg.render_ms * e_guided / e_plain
             run = {"guide_seed": gs, "train_ms": round(train_ms, 1), "train_batches": n, "adam_steps": steps,
                    "spp": spp_g, "render_ms": round(sg.render_ms, 1), "relmse": round(e_guided, 6),
                    "equal_time_relmse_ratio": round(e_guided / e_plain, 3),
                    "equal_spp_relmse_ratio": round(e_guided_eq / e_plain, 3),
                    "guided_over_plain_ms_per_spp": round(per_spp / (sp.render_ms / spp_plain), 3),
-                   "time_to_plain_relmse_ratio": round(t_match / budget, 3)}
+                   "time_to_plain_relmse": {"spp": spp_m, "render_ms": round(sm.render_ms, 1),
+                                            "relmse": round(e_m, 6), "relmse_over_plain": round(e_m / e_plain, 3),
+                                            "reached": bool(e_m <= e_plain)},
+                   "time_to_plain_relmse_ratio": round((train_ms + sm.render_ms) / budget, 3),
+                   "time_to_plain_relmse_ratio_extrapolated": round(t_pred / budget, 3)}
             log(f"guided: {json.dumps(run)}")
             runs.append(run)
 
@@ -396,6 +426,8 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
         "equal_spp_relmse_ratio": med("equal_spp_relmse_ratio"),
         "guided_over_plain_ms_per_spp": med("guided_over_plain_ms_per_spp"),
         "time_to_plain_relmse_ratio": med("time_to_plain_relmse_ratio"),
+        "time_to_plain_relmse_measured": "guided renders until relMSE <= plain's; training time counted",
+        "time_to_plain_relmse_ratio_extrapolated": med("time_to_plain_relmse_ratio_extrapolated"),
         "target_time_ratio": 0.5,
         "statistic": f"median over guide seeds {list(guide_seeds)}",
         "network_roofline": net,
@@ -547,6 +579,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-workload leg")
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-no-tune", action="store_true", help="batch leg with the default launch shape")
+    ap.add_argument("--batch-alone", action="store_true",
+                    help="batch leg without the per-job-stream and one-call-per-job comparisons (for profiles)")
     ap.add_argument("--batch-eager", action="store_true", help="batch leg without captured graphs")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided equal-time leg")
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
@@ -572,7 +606,7 @@ def main():
 
     if args.only:
         leg = {"batch": lambda: batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
-                                          not args.batch_eager),
+                                          not args.batch_eager, not args.batch_alone),
                "guided": lambda: guided_leg(pt, args.guided_ref_spp, args.guided_plain_spp),
                "guided-kernels": lambda: guided_kernels(pt),
                "configs": lambda: configs_leg(pt, args.cpu_seconds),

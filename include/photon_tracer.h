@@ -339,9 +339,14 @@ pt_status pt_device_copy(void* dst, const void* src, size_t bytes, int kind);
 /* ---------------------------------------------------------------------------
  * Batched rendering (the config-3 workload): many jobs -- scenes, views, image
  * sizes and spp budgets of uneven size -- in one call. The jobs' scenes are
- * merged into one device scene; each job runs pt_render's bounce loop over
- * its view of it, the jobs spread over a few HIP streams that run side by
- * side, each stream's whole job sequence one captured graph. The reference
+ * merged into one device scene. By default the jobs' paths share the ray
+ * queues: passes of the bounce loop each carry slices of many jobs (a job's
+ * pixels for a run of its samples), every kernel reading a path's camera,
+ * tree root, large spheres and lights from its slice, so the GPU stays full
+ * until the last pass; the whole sequence is one captured graph. With
+ * PT_BATCH_STREAMS each job runs pt_render's bounce loop over its view of
+ * the merged scene instead, the jobs spread over a few HIP streams that run
+ * side by side, each stream's job sequence one captured graph. The reference
  * renders one image per call (render_row_linear, renderer.hpp:411-416); a
  * caller with many jobs calls it once per job. Every job's image is bit for
  * bit what pt_render gives for the same scene, size, spp, seed and
@@ -381,33 +386,42 @@ typedef struct {
     float camera[16];           /* with PT_JOB_CAMERA: origin, corner, across, up (4 floats each, w unused) */
 } pt_batch_job;
 
+/* pt_batch_params.flags: each job's own bounce loops, jobs spread over HIP
+ * streams, instead of shared-queue passes (the default). */
+enum { PT_BATCH_STREAMS = 0x200 };
+
 typedef struct {
     int32_t max_bounces;        /* every job's bounce cap, 1 .. 63 */
-    int32_t flags;              /* PT_RENDER_NO_GRAPH, PT_RENDER_DEVICE_OUTPUT, PT_RENDER_NO_TAIL */
+    int32_t flags;              /* PT_RENDER_NO_GRAPH, PT_RENDER_DEVICE_OUTPUT, PT_RENDER_NO_TAIL, PT_BATCH_STREAMS */
     int32_t stack_entries;      /* 8, 16 or 32; 0 = tuned (pt_batch_autotune) or 8 */
     int32_t blocks_per_cu;      /* traversal blocks per CU; 0 = tuned or the occupancy limit */
     int32_t streams;            /* HIP streams the jobs are spread over, 1 .. 8; 0 = tuned or 2 */
     int32_t reserved;
-    uint64_t max_paths;         /* paths one batch of a job may hold (the queue size); 0 = tuned or memory-sized */
+    uint64_t max_paths;         /* paths one pass (or, with PT_BATCH_STREAMS, one batch of a job) may hold: the
+                                   queue size, at most 2^28 (2^27); 0 = tuned or memory-sized */
 } pt_batch_params;
 
 /* Renders every job. rgb: the jobs' images one after another in job order,
  * each 3 floats a pixel, top row first (a host array, or with
- * PT_RENDER_DEVICE_OUTPUT a device one). A job's samples go in batches of
- * spb samples a pixel, spb the largest divisor of its spp whose batch fits
- * max_paths; a pixel's samples are summed in sample order, so the batch size
- * changes no bit of the result. stats: rays as pt_render counts them, summed
- * over the jobs; batches: summed over the jobs; render_ms: first stream start
- * to last stream end. */
+ * PT_RENDER_DEVICE_OUTPUT a device one). Shared queues: passes of at most
+ * max_paths paths, filled with the jobs in order, each job's samples in
+ * order and split where a pass fills. PT_BATCH_STREAMS: a job's samples go
+ * in batches of spb samples a pixel, spb the largest divisor of its spp whose
+ * batch fits max_paths. Either way a pixel's samples are summed in sample
+ * order, so the split changes no bit of the result. stats: rays as pt_render
+ * counts them, summed over the jobs; batches: passes (shared queues) or
+ * batches summed over the jobs; render_ms: device time of the whole call. */
 pt_status pt_render_batch(pt_batch* batch, const pt_batch_job* jobs, int n_jobs, const pt_batch_params* params,
                           float* rgb, pt_render_stats* stats);
 
-/* The autotuner: renders the jobs (as given: pass fewer samples for a quicker
- * tune) under candidate launch shapes -- LDS stack entries (8, 16, 32) x
- * traversal blocks per CU, then streams (1, 2, 4), then batch sizes (2^22,
- * 2^24 paths) -- and keeps the fastest in the batch for later renders whose
- * params leave them 0. result (optional): the chosen shape and the measured
- * rate of each candidate. */
+/* The autotuner: renders the jobs in shared-queue passes (as given: pass
+ * fewer samples for a quicker tune) under candidate launch shapes -- LDS
+ * stack entries (8, 16, 32) x traversal blocks per CU, then pass sizes
+ * (2^25, 2^26, 2^27 paths, and the memory-sized default) -- and keeps the
+ * fastest in the batch for later renders whose params leave them 0 (streams
+ * stay untuned: 2 with PT_BATCH_STREAMS).
+ * result (optional): the chosen shape and the measured rate of each
+ * candidate. */
 typedef struct {
     int32_t stack_entries, blocks_per_cu, streams, n_tried;
     uint64_t max_paths;

@@ -42,6 +42,7 @@ RENDER_TRAIN = 128
 GUIDE_PARAMS = 18720
 GUIDE_OUTPUTS = 32
 JOB_CAMERA = 0x100
+BATCH_STREAMS = 0x200
 
 # Algorithmic bytes of one traversal query (the ray in, the hit out, and every
 # record the reference's walk must read): 32 + 16 per query, 128 per node slab
@@ -734,18 +735,21 @@ class Batch:
             pass
 
     def render(self, jobs, max_bounces: int = 10, graph: bool = True, tail: bool = True, stack_entries: int = 0,
-               blocks_per_cu: int = 0, streams: int = 0, max_paths: int = 0, device_out: "DeviceBuffer | None" = None):
+               blocks_per_cu: int = 0, streams: int = 0, max_paths: int = 0, device_out: "DeviceBuffer | None" = None,
+               shared: bool = True):
         """Renders every job. Returns (images, stats): one (height, width, 3) array a job, top row first
-        (None with device_out: the images stay in that device buffer, one after another)."""
+        (None with device_out: the images stay in that device buffer, one after another). shared: the jobs'
+        paths in shared-queue passes; False: each job's own bounce loops over `streams` HIP streams."""
         arr = _job_array(jobs)
         flags = (0 if graph else RENDER_NO_GRAPH) | (0 if tail else RENDER_NO_TAIL) | \
-            (RENDER_DEVICE_OUTPUT if device_out is not None else 0)
+            (RENDER_DEVICE_OUTPUT if device_out is not None else 0) | (0 if shared else BATCH_STREAMS)
         p = _BatchParams(int(max_bounces), flags, int(stack_entries), int(blocks_per_cu), int(streams), 0,
                          int(max_paths))
         sizes = [int(j.width) * int(j.height) for j in jobs]
         st = _RenderStats()
         if device_out is not None:
-            assert device_out.nbytes >= 12 * sum(sizes)
+            if device_out.nbytes < 12 * sum(sizes):
+                raise ValueError("device_out holds fewer than 12 bytes per pixel of the jobs")
             _check(library().pt_render_batch(self._h, arr, len(jobs), ctypes.byref(p), ctypes.c_void_p(device_out.ptr),
                                              ctypes.byref(st)))
             stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,

@@ -35,6 +35,7 @@ struct batch_stream {
     hipGraphExec_t exec = nullptr;
     std::vector<ptd::FrameParams> key;   // the launches the graph was captured for
     int S = 0, trace_grid = 0, shade_grid = 0, batches_key = 0;
+    bool pool_key = false;               // the graph is a shared-queue sequence
     const float* out_key = nullptr;   // the output the graph's resolves write
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -58,12 +59,12 @@ struct pt_batch {
     pt_scene* scene = nullptr;
     std::vector<pt::merge_part> parts;
     std::vector<batch_stream> streams;
-    dbuf frames, totals, accum, out;
+    dbuf frames, totals, accum, out, pool_tab;
     int tuned_stack = 0, tuned_blocks = 0, tuned_streams = 0;
     uint64_t tuned_paths = 0;
     ~pt_batch() {
         for (auto& s : streams) s.release();
-        for (dbuf* b : {&frames, &totals, &accum, &out}) b->release();
+        for (dbuf* b : {&frames, &totals, &accum, &out, &pool_tab}) b->release();
         if (scene) (void)pt_scene_destroy(scene);
     }
 };
@@ -138,13 +139,14 @@ ptd::SceneView job_view(const pt_batch* b, const pt_batch_job& J) {
     return v;
 }
 
-// Paths a stream's batch may hold when the caller and the tuner leave it
-// open: a quarter of the free HBM shared by the streams, at most kMaxPaths.
-uint64_t default_paths(pt_batch* b, int n_streams) {
+// Paths a stream's batch (or a shared-queue pass) may hold when the caller
+// and the tuner leave it open: a quarter of the free HBM shared by the
+// streams, at most max_paths.
+uint64_t default_paths(pt_batch* b, int n_streams, uint64_t max_paths) {
     size_t free_bytes = 0, total_bytes = 0;
     HIPC(hipMemGetInfo(&free_bytes, &total_bytes));
     for (auto& s : b->streams) free_bytes += s.rb.bytes();
-    return std::max<uint64_t>(1, std::min<uint64_t>(kMaxPaths, free_bytes / 4 / kPathBytes / n_streams));
+    return std::max<uint64_t>(1, std::min<uint64_t>(max_paths, free_bytes / 4 / kPathBytes / n_streams));
 }
 
 struct render_shape {
@@ -152,9 +154,15 @@ struct render_shape {
     uint64_t cap;
 };
 
+// Paths one shared-queue pass may hold: every job shares one set of queues,
+// so they may take twice a single render's (2^28 paths, 52 GB of queues and
+// path state).
+constexpr uint64_t kMaxPoolPaths = uint64_t(kMaxPaths) * 2;
+
 render_shape shape_for(pt_batch* b, const pt_batch_params& P, const pt_batch_job* jobs, int n_jobs) {
     render_shape r{};
     pt_scene* s = b->scene;
+    const bool pool = !(P.flags & PT_BATCH_STREAMS);
     r.S = ptapi::stack_entries_for(s, P.stack_entries > 0 ? P.stack_entries : b->tuned_stack);
     r.per_cu = P.blocks_per_cu > 0 ? P.blocks_per_cu
                : b->tuned_blocks > 0 ? b->tuned_blocks
@@ -163,8 +171,13 @@ render_shape shape_for(pt_batch* b, const pt_batch_params& P, const pt_batch_job
     r.n_streams = P.streams > 0 ? P.streams : b->tuned_streams > 0 ? b->tuned_streams : std::min(2, n_jobs);
     r.n_streams = std::max(1, std::min({r.n_streams, n_jobs, kMaxStreams}));
     if (P.streams > kMaxStreams) throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: more than 8 streams");
-    r.cap = P.max_paths ? P.max_paths : b->tuned_paths ? b->tuned_paths : default_paths(b, r.n_streams);
-    if (r.cap > kMaxPaths) throw pt::status_error(PT_ERR_LIMIT, "pt_render_batch: max_paths above 2^27");
+    const uint64_t max_cap = pool ? kMaxPoolPaths : kMaxPaths;
+    r.cap = P.max_paths ? P.max_paths
+            : b->tuned_paths ? b->tuned_paths
+            : std::min<uint64_t>(max_cap, default_paths(b, pool ? 1 : r.n_streams, max_cap));
+    if (r.cap > max_cap)
+        throw pt::status_error(PT_ERR_LIMIT, pool ? "pt_render_batch: max_paths above 2^28"
+                                                  : "pt_render_batch: max_paths above 2^27 (per-job streams)");
     for (int j = 0; j < n_jobs; ++j) {
         const uint64_t np = uint64_t(jobs[j].width) * uint64_t(jobs[j].height);
         if (np > r.cap) {
@@ -173,6 +186,259 @@ render_shape shape_for(pt_batch* b, const pt_batch_params& P, const pt_batch_job
         }
     }
     return r;
+}
+
+// A slice of a shared-queue pass: samples [s0, s0 + ns) of every pixel of job j.
+struct slice_plan {
+    int job, s0, ns;
+};
+
+// Shared-queue passes of at most cap paths: the jobs in order, each job's
+// samples in order, split where a pass fills up, so a job's slices come in
+// sample order across passes (and every pixel of a slice in one pass).
+std::vector<std::vector<slice_plan>> plan_passes(const pt_batch_job* jobs, int n_jobs, const std::vector<job_plan>& jp,
+                                                 uint64_t cap) {
+    std::vector<std::vector<slice_plan>> passes(1);
+    uint64_t room = cap;
+    for (int j = 0; j < n_jobs; ++j) {
+        const uint64_t npix = jp[j].npix;
+        int s0 = 0;
+        while (s0 < jobs[j].spp) {
+            const uint64_t fit = room / npix;
+            if (fit == 0) {   // npix <= cap (shape_for), so a fresh pass holds at least one sample
+                passes.emplace_back();
+                room = cap;
+                continue;
+            }
+            const int ns = static_cast<int>(std::min<uint64_t>(uint64_t(jobs[j].spp - s0), fit));
+            passes.back().push_back({j, s0, ns});
+            room -= uint64_t(ns) * npix;
+            s0 += ns;
+        }
+    }
+    return passes;
+}
+
+// Every job in shared-queue passes on one stream: the pass's slices of all
+// jobs go through one bounce loop, so the GPU stays full until the last pass
+// and the queue tails of small jobs overlap with everything else. A job's
+// image is pt_render's bit for bit: each path draws from its own (seed,
+// pixel, sample) streams, and each pixel's samples are summed in sample
+// order. The whole sequence -- clear, passes, the jobs' resolves -- is one
+// captured graph, kept for later renders of the same plan; the slice tables
+// it reads are uploaded before each launch.
+void render_pool(pt_batch* b, const pt_batch_job* jobs, int n_jobs, const pt_batch_params& P, const render_shape& sh,
+                 float* rgb, pt_render_stats* stats) {
+    pt_scene* s = b->scene;
+    const int trace_grid = s->cus * sh.per_cu;
+    const int shade_grid = std::max(1, s->cus * 2);
+    std::vector<job_plan> jp(n_jobs);
+    size_t total_pix = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        jp[j].npix = size_t(jobs[j].width) * size_t(jobs[j].height);
+        jp[j].pix_off = total_pix;
+        total_pix += jp[j].npix;
+    }
+    if (total_pix >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render_batch: too many pixels");
+    const std::vector<std::vector<slice_plan>> passes = plan_passes(jobs, n_jobs, jp, sh.cap);
+
+    // The slice tables of every pass in one upload: per pass its slices,
+    // then the first paths and first pixels of its slices (n + 1 each).
+    std::vector<size_t> tab_off(passes.size());
+    size_t tab_bytes = 0, max_paths = 1;
+    for (size_t t = 0; t < passes.size(); ++t) {
+        tab_off[t] = tab_bytes;
+        const size_t n = passes[t].size();
+        tab_bytes += n * sizeof(ptd::PoolSlice) + ((2 * (n + 1) * 4 + 127) & ~size_t(127));
+    }
+    std::vector<unsigned char> tab(tab_bytes, 0);
+    const uint32_t n_nodes = static_cast<uint32_t>(s->node_map.size());
+    int max_ns = 0;
+    std::vector<uint32_t> pass_paths(passes.size()), pass_pix(passes.size());
+    for (size_t t = 0; t < passes.size(); ++t) {
+        const size_t n = passes[t].size();
+        auto* sl = reinterpret_cast<ptd::PoolSlice*>(tab.data() + tab_off[t]);
+        auto* spath = reinterpret_cast<uint32_t*>(tab.data() + tab_off[t] + n * sizeof(ptd::PoolSlice));
+        uint32_t* spix = spath + n + 1;
+        uint64_t path = 0, pix = 0;
+        for (size_t k = 0; k < n; ++k) {
+            const slice_plan& q = passes[t][k];
+            const pt_batch_job& J = jobs[q.job];
+            const ptd::SceneView v = job_view(b, J);
+            ptd::PoolSlice& o = sl[k];
+            o = ptd::PoolSlice{};
+            for (int r = 0; r < 4; ++r) o.cam[r] = v.cam[r];
+            o.seed = J.seed;
+            o.W = static_cast<uint32_t>(J.width);
+            o.H = static_cast<uint32_t>(J.height);
+            o.ns = static_cast<uint32_t>(q.ns);
+            o.index0 = static_cast<uint32_t>(J.first_sample + q.s0);
+            o.root = v.flat_root;
+            o.large_off = static_cast<uint32_t>(v.large - s->view.large);
+            o.n_large = v.n_large;
+            o.lights_off = static_cast<uint32_t>(v.lights - s->view.lights);
+            o.n_lights = v.n_lights;
+            o.flags = v.flags;
+            o.pix_off = static_cast<uint32_t>(jp[q.job].pix_off);
+            spath[k] = static_cast<uint32_t>(path);
+            spix[k] = static_cast<uint32_t>(pix);
+            // What the kernels rely on, checked before anything is launched:
+            // every index a slice leads to lies in its array.
+            const bool ok = q.ns >= 1 && J.width >= 2 && J.height >= 2 &&
+                            (o.root == NONE || o.root < n_nodes) &&
+                            uint64_t(o.large_off) + o.n_large <= s->view.n_large &&
+                            uint64_t(o.lights_off) + o.n_lights <= s->view.n_lights &&
+                            uint64_t(o.pix_off) + jp[q.job].npix <= total_pix;
+            if (!ok) throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: internal error in a slice table");
+            path += uint64_t(jp[q.job].npix) * uint64_t(q.ns);
+            pix += jp[q.job].npix;
+            max_ns = std::max(max_ns, q.ns);
+        }
+        if (n == 0 || path > sh.cap || path >= (uint64_t(1) << 32))
+            throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: internal error in a pass plan");
+        spath[n] = static_cast<uint32_t>(path);
+        spix[n] = static_cast<uint32_t>(pix);
+        pass_paths[t] = static_cast<uint32_t>(path);
+        pass_pix[t] = static_cast<uint32_t>(pix);
+        max_paths = std::max<size_t>(max_paths, path);
+    }
+
+    // Buffers: the first stream's queues and path state, sized for the
+    // largest pass; the jobs' sums, one set of counters and frame constants.
+    if (b->streams.empty()) b->streams.resize(1);
+    batch_stream& st = b->streams[0];
+    if (!st.stream) HIPC(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking));
+    ptapi::render_bufs& B = st.rb;
+    for (int q = 0; q < 2; ++q) {
+        B.ro[q].ensure(max_paths * 16);
+        B.rd[q].ensure(max_paths * 16);
+        B.rpath[q].ensure(max_paths * 4);
+    }
+    B.hits.ensure(max_paths * sizeof(ptd::Hit));
+    B.so.ensure(max_paths * 16);
+    B.sd.ensure(max_paths * 16);
+    B.sl.ensure(max_paths * 16);
+    B.pa.ensure(max_paths * 16);
+    B.pb.ensure(max_paths * 16);
+    B.pkey.ensure(max_paths * 8);
+    B.pq.ensure(max_paths * 8);
+    st.ctrl.ensure(sizeof(ptd::Ctrl) * (ptd::MAX_BOUNCES + 1));
+    st.spill.ensure(size_t(trace_grid) * ptd::TRACE_BLOCK * size_t(s->max_stack) * sizeof(uint2));
+    b->frames.ensure(sizeof(ptd::FrameConst));
+    b->totals.ensure(4 * sizeof(unsigned long long));
+    b->accum.ensure(total_pix * 16);
+    b->out.ensure(total_pix * 12);
+    b->pool_tab.ensure(std::max<size_t>(tab_bytes, 128));
+    const ptd::FrameConst fc0{0, 0, 0u, ~0u, 0u};
+    HIPC(hipMemcpy(b->frames.p, &fc0, sizeof fc0, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(b->pool_tab.p, tab.data(), tab_bytes, hipMemcpyHostToDevice));
+
+    const bool device_out = (P.flags & PT_RENDER_DEVICE_OUTPUT) != 0;
+    float* out = device_out ? rgb : b->out.as<float>();
+    ptd::FrameParams base{};
+    base.sc = s->view;
+    base.depth = P.max_bounces;
+    base.spb = 1;
+    for (int q = 0; q < 2; ++q) {
+        base.ro[q] = B.ro[q].as<float4>();
+        base.rd[q] = B.rd[q].as<float4>();
+        base.rpath[q] = B.rpath[q].as<uint32_t>();
+    }
+    base.hits = B.hits.as<ptd::Hit>();
+    base.so = B.so.as<float4>();
+    base.sd = B.sd.as<float4>();
+    base.sl = B.sl.as<float4>();
+    base.pa = B.pa.as<float4>();
+    base.pb = B.pb.as<float4>();
+    base.pkey = B.pkey.as<uint2>();
+    base.pq = B.pq.as<uint2>();
+    base.accum = b->accum.as<float4>();
+    base.ctrl = st.ctrl.as<ptd::Ctrl>();
+    base.frame = b->frames.as<ptd::FrameConst>();
+    base.totals = b->totals.as<unsigned long long>();
+    base.spill = st.spill.as<uint2>();
+    base.tail_threshold = (P.flags & PT_RENDER_NO_TAIL) ? 0u : ptapi::tail_threshold(trace_grid, false);
+    // The launches: clear every sum, the passes, each job's resolve.
+    std::vector<ptd::FrameParams> launches;
+    ptd::FrameParams clear = base;
+    clear.npix = static_cast<uint32_t>(total_pix);
+    launches.push_back(clear);
+    for (size_t t = 0; t < passes.size(); ++t) {
+        ptd::FrameParams fp = base;
+        const size_t n = passes[t].size();
+        const unsigned char* at = b->pool_tab.as<unsigned char>() + tab_off[t];
+        fp.slices = reinterpret_cast<const ptd::PoolSlice*>(at);
+        fp.slice_path = reinterpret_cast<const uint32_t*>(at + n * sizeof(ptd::PoolSlice));
+        fp.slice_pix = fp.slice_path + n + 1;
+        fp.n_slices = static_cast<uint32_t>(n);
+        fp.P = pass_paths[t];
+        fp.npix = pass_pix[t];
+        launches.push_back(fp);
+    }
+    for (int j = 0; j < n_jobs; ++j) {
+        ptd::FrameParams r = base;
+        r.accum = b->accum.as<float4>() + jp[j].pix_off;
+        r.npix = static_cast<uint32_t>(jp[j].npix);
+        r.spp = jobs[j].spp;
+        launches.push_back(r);
+    }
+    auto enqueue = [&]() -> hipError_t {
+        hipError_t e = ptd::launch_clear_frame(launches[0], st.stream);
+        for (size_t t = 0; e == hipSuccess && t < passes.size(); ++t)
+            e = ptd::launch_pool_pass(launches[1 + t], sh.S, trace_grid, shade_grid, st.stream);
+        for (int j = 0; e == hipSuccess && j < n_jobs; ++j)
+            e = ptd::launch_resolve(launches[1 + passes.size() + j], out + 3 * jp[j].pix_off, nullptr, st.stream);
+        return e;
+    };
+    const bool graph = !(P.flags & PT_RENDER_NO_GRAPH);
+    if (graph) {
+        const bool reuse = st.exec && st.pool_key && st.S == sh.S && st.trace_grid == trace_grid &&
+                           st.shade_grid == shade_grid && st.out_key == out && st.key.size() == launches.size() &&
+                           std::memcmp(st.key.data(), launches.data(), launches.size() * sizeof(ptd::FrameParams)) == 0;
+        if (!reuse) {
+            st.drop_graph();
+            hipGraph_t g;
+            HIPC(hipStreamBeginCapture(st.stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t le = enqueue();
+            const hipError_t ce = hipStreamEndCapture(st.stream, &g);
+            HIPC(le);
+            HIPC(ce);
+            const hipError_t ie = hipGraphInstantiate(&st.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) st.exec = nullptr;
+            HIPC(ie);
+            st.key = launches;
+            st.pool_key = true;
+            st.S = sh.S;
+            st.trace_grid = trace_grid;
+            st.shade_grid = shade_grid;
+            st.out_key = out;
+        }
+    }
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, st.stream));
+    if (graph) HIPC(hipGraphLaunch(st.exec, st.stream));
+    else HIPC(enqueue());
+    HIPC(hipEventRecord(e1, st.stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!device_out) HIPC(hipMemcpy(rgb, b->out.p, total_pix * 12, hipMemcpyDeviceToHost));
+    unsigned long long tot[4] = {};
+    HIPC(hipMemcpy(tot, b->totals.p, sizeof tot, hipMemcpyDeviceToHost));
+    if (stats) {
+        *stats = pt_render_stats{};
+        stats->rays = tot[0] + tot[1];
+        stats->shadow_rays = tot[1];
+        stats->primary_rays = tot[2];
+        stats->batches = static_cast<int32_t>(passes.size());
+        stats->samples_per_batch = max_ns;
+        stats->render_ms = ms;
+    }
 }
 
 }  // namespace
@@ -236,10 +502,15 @@ pt_status pt_render_batch(pt_batch* b, const pt_batch_job* jobs, int n_jobs, con
         pt_scene* s = b->scene;
         if (P.max_bounces < 1 || P.max_bounces >= ptd::MAX_BOUNCES)
             throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: max_bounces must be in [1, 63]");
-        if (P.flags & ~(PT_RENDER_NO_GRAPH | PT_RENDER_DEVICE_OUTPUT | PT_RENDER_NO_TAIL))
-            throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: flags other than NO_GRAPH, DEVICE_OUTPUT, NO_TAIL");
+        if (P.flags & ~(PT_RENDER_NO_GRAPH | PT_RENDER_DEVICE_OUTPUT | PT_RENDER_NO_TAIL | PT_BATCH_STREAMS))
+            throw pt::status_error(PT_ERR_INVALID,
+                                   "pt_render_batch: flags other than NO_GRAPH, DEVICE_OUTPUT, NO_TAIL, BATCH_STREAMS");
         check_jobs(b, jobs, n_jobs);
         const render_shape sh = shape_for(b, P, jobs, n_jobs);
+        if (!(P.flags & PT_BATCH_STREAMS)) {
+            render_pool(b, jobs, n_jobs, P, sh, rgb, stats);
+            return PT_OK;
+        }
         const int trace_grid = s->cus * sh.per_cu;
         const int shade_grid = std::max(1, s->cus * 2);
 
@@ -367,7 +638,7 @@ pt_status pt_render_batch(pt_batch* b, const pt_batch_job* jobs, int n_jobs, con
                 batch_stream& st = b->streams[k];
                 int batches = 0;
                 for (int j : per_stream[k]) batches = batches * 131 + jp[j].batches;
-                const bool reuse = st.exec && st.S == sh.S && st.trace_grid == trace_grid &&
+                const bool reuse = st.exec && !st.pool_key && st.S == sh.S && st.trace_grid == trace_grid &&
                                    st.shade_grid == shade_grid && st.batches_key == batches && st.out_key == out &&
                                    st.key.size() == fps[k].size() &&
                                    std::memcmp(st.key.data(), fps[k].data(), fps[k].size() * sizeof(ptd::FrameParams)) == 0;
@@ -384,6 +655,7 @@ pt_status pt_render_batch(pt_batch* b, const pt_batch_job* jobs, int n_jobs, con
                 if (ie != hipSuccess) st.exec = nullptr;
                 HIPC(ie);
                 st.key = fps[k];
+                st.pool_key = false;
                 st.S = sh.S;
                 st.trace_grid = trace_grid;
                 st.shade_grid = shade_grid;
@@ -468,7 +740,8 @@ pt_status pt_batch_autotune(pt_batch* b, const pt_batch_job* jobs, int n_jobs, i
         auto best = [&] {
             return *std::max_element(tried.begin(), tried.end(), [](const cand& x, const cand& y) { return x.mrays < y.mrays; });
         };
-        // Stage 1: LDS stack entries x traversal blocks per CU, one stream.
+        // Stage 1: LDS stack entries x traversal blocks per CU (shared-queue
+        // passes, which use one stream).
         for (int S : {8, 16, 32}) {
             const int occ = ptd::path_blocks_per_cu(S, false);
             std::vector<int> blocks{occ};
@@ -477,16 +750,11 @@ pt_status pt_batch_autotune(pt_batch* b, const pt_batch_job* jobs, int n_jobs, i
             for (int pc : blocks) rate(S, pc, 1, 0);
         }
         cand c = best();
-        // Stage 2: streams side by side.
-        for (int n : {2, 4})
-            if (n <= n_jobs) rate(c.S, c.per_cu, n, 0);
-        c = best();
-        // Stage 3: smaller batches (queue sizes) at the best shape.
-        for (int e : {22, 24}) rate(c.S, c.per_cu, c.n_streams, uint64_t(1) << e);
+        // Stage 2: smaller passes (queue sizes) at the best shape.
+        for (int e : {25, 26, 27}) rate(c.S, c.per_cu, 1, uint64_t(1) << e);
         c = best();
         b->tuned_stack = c.S;
         b->tuned_blocks = c.per_cu;
-        b->tuned_streams = c.n_streams;
         b->tuned_paths = c.paths;
         if (result) {
             *result = pt_tune_result{};

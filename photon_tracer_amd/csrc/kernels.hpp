@@ -41,6 +41,26 @@ struct FrameConst {
     uint32_t pad;
 };
 
+// Shared-queue batches: one pass of the bounce loop carries the paths of
+// many jobs at once, in slices. A slice is one job's pixels for a run of its
+// samples (ns a pixel from sample index index0 on); its paths are numbered
+// pixel-major from path_base, as one render numbers a batch's. Everything a
+// kernel reads of a job comes from its path's slice, found from the path
+// number by slice_of's search, whose result is a valid slice for any number.
+struct PoolSlice {
+    float4 cam[4];          // origin, corner, across, up
+    uint64_t seed;
+    uint32_t W, H;          // the job's image size
+    uint32_t ns, index0;    // samples a pixel in this slice, the first one's number
+    uint32_t root;          // the job's tree root (device node), or NONE
+    uint32_t large_off, n_large;    // its run of the merged large-sphere list
+    uint32_t lights_off, n_lights;  // its run of the merged light list
+    uint32_t flags;         // 1 sky, 2 sample lights (the job's NO_NEE applied)
+    uint32_t pix_off;       // the job's first pixel in the sums
+    uint32_t pad[3];
+};
+static_assert(sizeof(PoolSlice) == 128, "slice records are 128 bytes");
+
 struct FrameParams {
     SceneView sc;
     int W, H, spp, depth;
@@ -86,6 +106,13 @@ struct FrameParams {
     float gclamp;             // training weights are clamped at gclamp x their batch mean
     int guide_bounces;        // diffuse vertices of bounces [0, guide_bounces) sample the guide
     float* gmis;              // guided renders: per path, the product of its guided MIS factors
+    // Shared-queue passes (0 slices: a single render): n_slices slices, their
+    // first paths (slice_path[0] = 0, increasing, slice_path[n_slices] = P)
+    // and first pass pixels (slice_pix, the same shape, ending at npix).
+    const PoolSlice* slices;
+    const uint32_t* slice_path;
+    const uint32_t* slice_pix;
+    uint32_t n_slices;
 };
 constexpr uint32_t GUIDE_RECORDS = 2;   // diffuse vertices per path recorded for training
 constexpr uint32_t GUIDE_TARGET_BLOCK = 256;
@@ -106,6 +133,12 @@ constexpr int TRACE_BLOCK = 256;
 hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid, hipStream_t stream,
                         hipEvent_t* ev);
 hipError_t launch_resolve(const FrameParams& fp, float* rgb, float* aux, hipStream_t stream);
+// One shared-queue pass: the bounce loop over every slice's paths (no
+// instances, guide or aux), then each pass pixel's samples added onto its
+// job's sums. The caller clears the sums first (launch_clear_frame over all
+// of them) and resolves each job after the last pass.
+hipError_t launch_pool_pass(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid,
+                            hipStream_t stream);
 hipError_t launch_progressive_add(const float* img, float* sum, uint32_t npix, int n, bool first, hipStream_t stream);
 hipError_t launch_progressive_resolve(const float* sum, float* rgb, uint32_t npix, int done, hipStream_t stream);
 // After a training batch: its records' weights, compacted in path order into

@@ -128,6 +128,85 @@ __device__ __forceinline__ CameraSample camera_sample(const FrameParams& fp, con
     return cs;
 }
 
+// Shared-queue passes. The slice of pass path p (with slice_pix, of pass
+// pixel p): the last slice whose first entry is at most p. base[0] = 0, so
+// the answer lies in [0, n - 1] for any p, and only base[0, n) is read; the
+// host launches these kernels with n >= 1.
+__device__ __forceinline__ uint32_t slice_of(const uint32_t* base, uint32_t n, uint32_t p) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A lane's last slice and its path range: queues keep paths roughly in
+// path order, so the next path is mostly in the same slice.
+struct SliceCache {
+    uint32_t k = 0, lo = 1, hi = 0;   // an empty range: the first lookup searches
+    __device__ __forceinline__ uint32_t find(const FrameParams& fp, uint32_t p) {
+        if (!(p >= lo && p < hi)) {
+            k = slice_of(fp.slice_path, fp.n_slices, p);
+            lo = fp.slice_path[k];
+            hi = fp.slice_path[k + 1];   // the array holds n_slices + 1 entries
+        }
+        return k;
+    }
+};
+
+// Slice k for the lanes that ask; one read through the scalar unit when they
+// all ask for the same slice (a wave's paths are mostly one job's).
+__device__ __forceinline__ PoolSlice read_slice(const FrameParams& fp, uint32_t k) {
+    const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+    if (__ballot(k != k0) == 0) return fp.slices[k0];
+    return fp.slices[k];
+}
+
+// camera_sample for path pl of a slice (counted from the slice's first
+// path): the job's whole frame, its samples numbered from index0.
+__device__ __forceinline__ CameraSample camera_sample_pool(const PoolSlice& S, uint32_t pl) {
+    CameraSample cs;
+    const uint32_t pixel = pl / S.ns, s = pl % S.ns;
+    cs.i = static_cast<int>(pixel % S.W);
+    cs.j = static_cast<int>(S.H) - 1 - static_cast<int>(pixel / S.W);
+    cs.index = S.index0 + s;
+    cs.pseed = pixel_seed(S.seed, cs.i, cs.j);
+    float du, dv;
+    qmc_pair(cs.pseed, cs.index, 0, du, dv);
+    const float u = (static_cast<float>(cs.i) + du) / static_cast<float>(static_cast<int>(S.W) - 1);
+    const float v = (static_cast<float>(cs.j) + dv) / static_cast<float>(static_cast<int>(S.H) - 1);
+    cs.o = xyz(S.cam[0]);
+    cs.d = xyz(S.cam[1]) + xyz(S.cam[2]) * u + xyz(S.cam[3]) * v - cs.o;
+    return cs;
+}
+
+// The scene a slice's rays are traced against: the merged scene with the
+// job's tree root and large spheres.
+__device__ __forceinline__ SceneView slice_scene(const SceneView& sc, const PoolSlice& S) {
+    SceneView v = sc;
+    v.flat_root = S.root;
+    v.large = sc.large + S.large_off;
+    v.n_large = S.n_large;
+    return v;
+}
+
+// What shading reads of a job besides the shared arrays: the sky flag, NEE
+// and the lights NEE picks from.
+struct LightView {
+    uint32_t flags;
+    bool nee;
+    const uint32_t* lights;
+    uint32_t n_lights;
+};
+__device__ __forceinline__ LightView light_view(const FrameParams& fp) {
+    return LightView{fp.sc.flags, fp.nee != 0, fp.sc.lights, fp.sc.n_lights};
+}
+__device__ __forceinline__ LightView light_view(const FrameParams& fp, const PoolSlice& S) {
+    return LightView{S.flags, (S.flags & 2u) != 0, fp.sc.lights + S.lights_off, S.n_lights};
+}
+
 // An unoccluded light sample of bounce bs: its radiance joins its path and,
 // while the guide trains, the record of the vertex that took it, whose
 // radiance then counts as arrived before the vertex's scattered ray.
@@ -211,7 +290,7 @@ constexpr uint32_t SEGMENT_MIN = PT_SEGMENT_MIN;
 // in the same walk); false keeps the per-wave fetch of 64 queries for them.
 constexpr bool INST_REFILL = true;
 
-template <int S, bool COUNT, bool PRIMARY, bool INST, class Stack>
+template <int S, bool COUNT, bool PRIMARY, bool INST, bool POOL, class Stack>
 __device__ __forceinline__ void extend_refill(const FrameParams& fp, const SceneView& sc, int b, Ctrl* c,
                                               uint32_t n_shadow, uint32_t n, Stack& st, Visits& vis) {
     const float4* ro = fp.ro[b & 1];
@@ -243,6 +322,7 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, const Scene
     bool live = false, any = false;
     uint32_t item = 0;
     Walk<INST> w;
+    SliceCache slc;
     while (true) {
         const unsigned long long idle = __ballot(!live);
         const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
@@ -270,19 +350,42 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, const Scene
                     any = !PRIMARY && item < n_shadow;
                     f3 o, d;
                     float tmin = T_MIN, tmax = T_MAX;
-                    if (PRIMARY) {
-                        const CameraSample cs = camera_sample(fp, fc, item);
-                        o = cs.o;
-                        d = cs.d;
+                    if constexpr (POOL) {
+                        // The ray's path and slice: the job's camera, tree and large spheres.
+                        uint32_t p = item;
+                        if (!PRIMARY) {
+                            p = any ? reinterpret_cast<const uint32_t*>(fp.sl)[4 * size_t(item) + 3]
+                                    : fp.rpath[b & 1][item - n_shadow];
+                        }
+                        const PoolSlice sl = read_slice(fp, slc.find(fp, p));
+                        if (PRIMARY) {
+                            const CameraSample cs = camera_sample_pool(sl, item - slc.lo);
+                            o = cs.o;
+                            d = cs.d;
+                        } else {
+                            const float4 a = any ? fp.so[item] : ro[item - n_shadow];
+                            const float4 bb = any ? fp.sd[item] : rd[item - n_shadow];
+                            o = xyz(a);
+                            d = xyz(bb);
+                            tmin = a.w;
+                            tmax = bb.w;
+                        }
+                        live = !walk_begin<COUNT, INST>(slice_scene(sc, sl), w, o, d, tmin, tmax, any, st, &vis);
                     } else {
-                        const float4 a = any ? fp.so[item] : ro[item - n_shadow];
-                        const float4 bb = any ? fp.sd[item] : rd[item - n_shadow];
-                        o = xyz(a);
-                        d = xyz(bb);
-                        tmin = a.w;
-                        tmax = bb.w;
+                        if (PRIMARY) {
+                            const CameraSample cs = camera_sample(fp, fc, item);
+                            o = cs.o;
+                            d = cs.d;
+                        } else {
+                            const float4 a = any ? fp.so[item] : ro[item - n_shadow];
+                            const float4 bb = any ? fp.sd[item] : rd[item - n_shadow];
+                            o = xyz(a);
+                            d = xyz(bb);
+                            tmin = a.w;
+                            tmax = bb.w;
+                        }
+                        live = !walk_begin<COUNT, INST>(sc, w, o, d, tmin, tmax, any, st, &vis);
                     }
-                    live = !walk_begin<COUNT, INST>(sc, w, o, d, tmin, tmax, any, st, &vis);
                     if (!live) {   // ended at once: only an any-hit query on a large sphere
                         item |= 0x80000000u;
                     }
@@ -328,7 +431,7 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, const Scene
     }
 }
 
-template <int S, bool COUNT, bool INST, bool PRIMARY>
+template <int S, bool COUNT, bool INST, bool PRIMARY, bool POOL = false>
 __global__ __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void k_extend(FrameParams fp, int b) {
     if (!PRIMARY && fp.frame->tail_from <= static_cast<uint32_t>(b)) return;  // the tail launch has these rays
@@ -342,8 +445,8 @@ __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void
     const uint32_t n_rays = PRIMARY ? fp.P : c->n_rays;
     const uint32_t n = n_shadow + n_rays;
     if (PRIMARY && blockIdx.x == 0 && threadIdx.x == 0) c->n_rays = fp.P;
-    if constexpr (!INST || INST_REFILL) {
-        extend_refill<S, COUNT, PRIMARY, INST>(fp, sc, b, c, n_shadow, n, st, vis);
+    if constexpr (!INST || INST_REFILL || POOL) {
+        extend_refill<S, COUNT, PRIMARY, INST, POOL>(fp, sc, b, c, n_shadow, n, st, vis);
         if (COUNT) add_visits(fp.visits, vis);
         return;
     }
@@ -429,7 +532,7 @@ struct Surface {
     float tu, tv;   // texture coordinates (triangle.hpp:72-73; 0 on spheres, hittable.hpp:14)
 };
 
-__device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& h, f3 o, f3 d) {
+__device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& h, f3 o, f3 d, bool nee) {
     const SceneView& sc = fp.sc;
     Surface sf;
     const f3 pnt = o + d * h.t;
@@ -445,7 +548,7 @@ __device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& 
         const f3 outward = (pnt - xyz(cr)) * (1 / cr.w);
         front = dot(d, outward) < 0;
         nrm = front ? outward : -outward;
-        if (h.kind == 2 && fp.nee) {
+        if (h.kind == 2 && nee) {
             const float4 em = sc.mats[3 * mat + 1];
             sampled_light = em.x > 0 || em.y > 0 || em.z > 0;
         }
@@ -501,7 +604,7 @@ __device__ __forceinline__ void record_vertex(const FrameParams& fp, uint32_t p,
 // this hit when diffuse bounces sample the guide, else null.
 template <bool GUIDED = false>
 __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, uint32_t p, const Hit& h, f3 o, f3 d,
-                                                 PathState& ps, const _Float16* go) {
+                                                 PathState& ps, const _Float16* go, const LightView& lv) {
     const SceneView& sc = fp.sc;
     ShadeResult r;
     r.cont = r.shadow = r.backface = false;
@@ -512,7 +615,7 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
     if (h.kind == 0) {
         // renderer.hpp:24-29
         f3 bg = mk(0, 0, 0);
-        if (sc.flags & 1u) {
+        if (lv.flags & 1u) {
             const f3 unit = normalize(d);
             const float t = 0.5f * (unit.y + 1);
             bg = mk(1, 1, 1) * (1 - t) + mk(0.5f, 0.7f, 1.0f) * t;
@@ -524,7 +627,7 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
         }
         return r;
     }
-    const Surface sf = surface_at(fp, h, o, d);
+    const Surface sf = surface_at(fp, h, o, d, lv.nee);
     const f3 pnt = sf.pnt, nrm = sf.nrm;
     const bool front = sf.front, sampled_light = sf.sampled_light;
     const uint32_t mat = sf.mat;
@@ -604,12 +707,12 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
         fp.paux[2 * p + 1] = make_float4(nrm.x, nrm.y, nrm.z, 0);
     }
     if (!ok) return r;
-    const bool nee = type == 0 && fp.nee && sc.n_lights > 0 && b + 1 < fp.depth;
+    const bool nee = type == 0 && lv.nee && lv.n_lights > 0 && b + 1 < fp.depth;
     if (nee) {
         // direct_light (renderer.hpp:41-80)
-        const uint32_t nl = sc.n_lights;
+        const uint32_t nl = lv.n_lights;
         const uint32_t pick = min(static_cast<uint32_t>(rand01(ps.key, ps.cnt) * nl), nl - 1);
-        const uint32_t ls = sc.lights[pick];
+        const uint32_t ls = lv.lights[pick];
         const float4 lcr = sc.spheres[2 * ls];
         const uint32_t lmat = __float_as_uint(sc.spheres[2 * ls + 1].x);
         const f3 to_center = xyz(lcr) - pnt;
@@ -710,7 +813,7 @@ __global__ __launch_bounds__(INFER_BLOCK) __attribute__((amdgpu_waves_per_eu(4))
                     o = xyz(fp.ro[cur][q]);
                     d = xyz(fp.rd[cur][q]);
                 }
-                const Surface sf = surface_at(fp, h, o, d);
+                const Surface sf = surface_at(fp, h, o, d, fp.nee != 0);
                 diffuse = static_cast<int>(fp.sc.mats[3 * sf.mat].w) == 0;
                 pnt = sf.pnt;
                 nrm = sf.nrm;
@@ -760,7 +863,7 @@ __global__ __launch_bounds__(INFER_BLOCK) __attribute__((amdgpu_waves_per_eu(4))
 // ray and the fresh path state are rebuilt from the path number instead of
 // read back. GUIDED: diffuse vertices sample the guide (its own kernel, so
 // the plain one carries none of that code's registers).
-template <bool PRIMARY, bool GUIDED = false>
+template <bool PRIMARY, bool GUIDED = false, bool POOL = false>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     if (fp.frame->tail_from <= static_cast<uint32_t>(b)) return;  // the tail launch has taken these paths
     Ctrl* c = fp.ctrl + b;
@@ -777,6 +880,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     __shared__ uint32_t gtile[GUIDED ? SHADE_BLOCK * GROW : 1];
     uint32_t* const counters[3] = {&fp.ctrl[b + 1].n_rays, &c->n_shadow, &c->n_backface};
     const uint32_t stride = gridDim.x * SHADE_BLOCK;
+    SliceCache slc;
     // The block steps through the queue together (block_append synchronises).
     for (uint32_t bbase = blockIdx.x * SHADE_BLOCK; bbase < n; bbase += stride) {
         const uint32_t q = bbase + threadIdx.x;
@@ -801,9 +905,18 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
             const Hit h = fp.hits[q];
             f3 o, d;
             PathState ps;
+            LightView lv;
+            PoolSlice sl;
+            if (POOL) {
+                p = PRIMARY ? q : fp.rpath[cur][q];
+                sl = read_slice(fp, slc.find(fp, p));
+                lv = light_view(fp, sl);
+            } else {
+                lv = light_view(fp);
+            }
             if (PRIMARY) {
                 p = q;
-                const CameraSample cs = camera_sample(fp, fc, p);
+                const CameraSample cs = POOL ? camera_sample_pool(sl, p - slc.lo) : camera_sample(fp, fc, p);
                 o = cs.o;
                 d = cs.d;
                 ps.thr = mk(1, 1, 1);
@@ -812,7 +925,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 ps.sampled = false;
                 ps.nrec = 0;
                 ps.mis = 1;
-                ps.key = path_key(fc.seed, cs.i, cs.j, cs.index);
+                ps.key = path_key(POOL ? sl.seed : fc.seed, cs.i, cs.j, cs.index);
                 ps.pq = make_uint2(cs.pseed, cs.index);
             } else {
                 p = fp.rpath[cur][q];
@@ -829,7 +942,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
                 o = xyz(o4);
                 d = xyz(d4);
             }
-            r = shade_hit<GUIDED>(fp, b, p, h, o, d, ps, go);
+            r = shade_hit<GUIDED>(fp, b, p, h, o, d, ps, go, lv);
             // The radiance always (accumulate reads it); the rest only for a
             // path that goes on.
             fp.pb[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, path_flags(ps));
@@ -870,7 +983,7 @@ __device__ __forceinline__ bool tail_starts(const FrameParams& fp, int b) {
 
 // Tail, part 1: bounce b - 1's light samples, so that their radiance is in
 // place before part 2 carries the paths on.
-template <int S, bool INST>
+template <int S, bool INST, bool POOL = false>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_tail_shadow(FrameParams fp, int b) {
     if (!tail_starts(fp, b)) return;
     __shared__ uint2 lds[S * TRACE_BLOCK];
@@ -878,6 +991,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail_shadow(FrameParams fp, int
     const uint32_t n = fp.ctrl[b - 1].n_shadow;
     uint32_t* work = &fp.ctrl[MAX_BOUNCES].w_unused;
     const uint32_t groups = fetch_groups(n);
+    SliceCache slc;
     while (true) {
         const uint32_t base = wave_fetch(work, 64 * groups);
         if (base >= n) break;
@@ -886,7 +1000,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail_shadow(FrameParams fp, int
             if (item >= n) break;
             const float4 a = fp.so[item], bb = fp.sd[item];
             Hit h;
-            if (!trace_ray<true, false, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st)) {
+            bool hit;
+            if constexpr (POOL) {
+                const uint32_t p = reinterpret_cast<const uint32_t*>(fp.sl)[4 * size_t(item) + 3];
+                const SceneView v = slice_scene(fp.sc, read_slice(fp, slc.find(fp, p)));
+                hit = trace_ray<true, false, INST>(v, xyz(a), xyz(bb), a.w, bb.w, h, st);
+            } else {
+                hit = trace_ray<true, false, INST>(fp.sc, xyz(a), xyz(bb), a.w, bb.w, h, st);
+            }
+            if (!hit) {
                 light_sample_arrives(fp, item, b - 1);
             }
         }
@@ -907,7 +1029,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail_shadow(FrameParams fp, int
 // the serial chain of the longest paths' remaining bounces.)
 constexpr uint32_t TAIL_REFILL = 8;
 
-template <int S, bool INST, bool GUIDED = false>
+template <int S, bool INST, bool GUIDED = false, bool POOL = false>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
     if (!tail_starts(fp, b)) return;
     Ctrl* c = fp.ctrl + b;
@@ -927,6 +1049,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
     PathState ps;
     f3 o = mk(0, 0, 0), d = mk(0, 0, 1);
     float tmin = T_MIN, tmax = T_MAX;
+    // The lane's path's job: its scene and lights (the single render's otherwise).
+    SliceCache slc;
+    SceneView view = fp.sc;
+    LightView lv = light_view(fp);
     while (true) {
         const unsigned long long idle = __ballot(!live);
         const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
@@ -959,20 +1085,25 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_tail(FrameParams fp, int b) {
                     tmax = d4.w;
                     bb = b;
                     live = true;
+                    if constexpr (POOL) {
+                        const PoolSlice sl = read_slice(fp, slc.find(fp, p));
+                        view = slice_scene(fp.sc, sl);
+                        lv = light_view(fp, sl);
+                    }
                 }
             }
             pool = min(pool + n_idle, pool_end);
         }
         if (!live) continue;
         Hit h;
-        trace_ray<false, false, INST>(fp.sc, o, d, tmin, tmax, h, st);
+        trace_ray<false, false, INST>(view, o, d, tmin, tmax, h, st);
         if (bb > b) ++rays;  // bounce b's rays were counted when they were queued
-        const ShadeResult r = shade_hit<GUIDED>(fp, bb, p, h, o, d, ps, nullptr);   // never samples the guide
+        const ShadeResult r = shade_hit<GUIDED>(fp, bb, p, h, o, d, ps, nullptr, lv);   // never samples the guide
         if (r.backface) ++backfaces;
         if (r.shadow) {
             ++shadows;
             Hit hs;
-            if (!trace_ray<true, false, INST>(fp.sc, r.pnt, r.sh_d, 0.001f, r.sh_tmax, hs, st)) {
+            if (!trace_ray<true, false, INST>(view, r.pnt, r.sh_d, 0.001f, r.sh_tmax, hs, st)) {
                 ps.rad.x = ps.rad.x + r.sh_l.x;
                 ps.rad.y = ps.rad.y + r.sh_l.y;
                 ps.rad.z = ps.rad.z + r.sh_l.z;
@@ -1065,6 +1196,24 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams fp) {
             fp.aux_accum[2 * px] = a0;
             fp.aux_accum[2 * px + 1] = a1;
         }
+    }
+}
+
+// Shared-queue passes: per pass pixel (a slice's pixels, numbered after the
+// earlier slices'), the slice's samples added in sample order onto its job's
+// pixel sum; a job's slices come in sample order, pass after pass, so every
+// pixel sums its samples in the order a single render does.
+__global__ __launch_bounds__(256) void k_accumulate_pool(FrameParams fp) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < fp.npix; g += gridDim.x * blockDim.x) {
+        const uint32_t k = slice_of(fp.slice_pix, fp.n_slices, g);
+        const PoolSlice& sl = fp.slices[k];
+        const uint32_t local = g - fp.slice_pix[k];
+        const float4* src = fp.pb + fp.slice_path[k] + static_cast<size_t>(local) * sl.ns;
+        float4* dst = fp.accum + sl.pix_off + local;
+        const float4 acc = *dst;
+        f3 sum = xyz(acc);
+        for (uint32_t s = 0; s < sl.ns; ++s) sum = sum + xyz(src[s]);
+        *dst = make_float4(sum.x, sum.y, sum.z, 0);
     }
 }
 
@@ -1302,6 +1451,37 @@ hipError_t launch_batch(const FrameParams& fp, int stack_entries, int trace_grid
             if (inst) launch_loop<32, true>(fp, trace_grid, shade_grid, stream, ev);
             else launch_loop<32, false>(fp, trace_grid, shade_grid, stream, ev);
             break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int S>
+static void launch_pool_loop(const FrameParams& fp, int trace_grid, int shade_grid, hipStream_t stream) {
+    for (int b = 0; b < fp.depth; ++b) {
+        if (b > 0) {
+            k_tail_shadow<S, false, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            k_tail<S, false, false, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            k_extend<S, false, false, false, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            k_shade<false, false, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        } else {
+            k_extend<S, false, false, true, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            k_shade<true, false, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
+        }
+    }
+    k_accumulate_pool<<<shade_grid * 8, 256, 0, stream>>>(fp);
+    k_finish<<<1, 64, 0, stream>>>(fp);
+}
+
+hipError_t launch_pool_pass(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid,
+                            hipStream_t stream) {
+    if (fp.depth < 1 || fp.depth >= MAX_BOUNCES || fp.n_slices < 1 || !fp.slices || !fp.slice_path ||
+        !fp.slice_pix || fp.guide_mode != 0 || fp.want_aux || fp.sc.inst_root != NONE)
+        return hipErrorInvalidValue;
+    switch (stack_entries) {
+        case 8: launch_pool_loop<8>(fp, trace_grid, shade_grid, stream); break;
+        case 16: launch_pool_loop<16>(fp, trace_grid, shade_grid, stream); break;
+        case 32: launch_pool_loop<32>(fp, trace_grid, shade_grid, stream); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -1,9 +1,10 @@
 """Batched renders (pt_render_batch): many jobs -- different scenes, views,
 image sizes, spp, seeds -- in one call over one merged device scene. Every
 job's image must be bit for bit the image pt_render gives for that job alone
-(which the oracle and the reference fixtures pin), whatever the batch sizes,
-streams and launch shape, with or without the captured graphs, and after the
-autotuner."""
+(which the oracle and the reference fixtures pin), in shared-queue passes
+(slices of many jobs in one bounce loop) or per-job loops over streams,
+whatever the pass or batch sizes, streams and launch shape, with or without
+the captured graphs, and after the autotuner."""
 import numpy as np
 import pytest
 
@@ -60,10 +61,12 @@ def singles():
     return sc, jobs, [single(sc, j) for j in jobs]
 
 
-def test_batch_equals_single_renders(singles):
+@pytest.mark.parametrize("shared", [True, False])
+def test_batch_equals_single_renders(singles, shared):
+    """Shared-queue passes (the default) and per-job bounce loops over streams."""
     sc, jobs, ref = singles
     with pt.Batch(sc) as b:
-        imgs, st = b.render(jobs)
+        imgs, st = b.render(jobs, shared=shared)
     for k, (img, (want, _)) in enumerate(zip(imgs, ref)):
         assert img.shape == want.shape
         same = np.array_equal(img.view(np.uint32), want.view(np.uint32))
@@ -74,30 +77,35 @@ def test_batch_equals_single_renders(singles):
     assert st.shadow_rays == sum(s.shadow_rays for _, s in ref)
 
 
-@pytest.mark.parametrize("max_paths,streams", [(6000, 1), (40000, 3), (1 << 20, 2), (1 << 20, 8)])
-def test_batch_sizes_and_streams_change_nothing(singles, max_paths, streams):
-    """Small queues split every job into many batches; jobs spread over several streams; eager launches and
-    the graphs agree."""
+@pytest.mark.parametrize("max_paths,streams,shared", [(6000, 1, False), (40000, 3, False), (1 << 20, 2, False),
+                                                      (1 << 20, 8, False), (6000, 1, True), (9000, 1, True),
+                                                      (40000, 1, True), (1 << 20, 1, True)])
+def test_batch_sizes_and_streams_change_nothing(singles, max_paths, streams, shared):
+    """Small queues split every job into many batches (per-job loops) or passes whose slices split jobs
+    between them (shared queues); jobs spread over several streams; eager launches and the graphs agree."""
     sc, jobs, ref = singles
     with pt.Batch(sc) as b:
-        a, sa = b.render(jobs, max_paths=max_paths, streams=streams)
-        e, se = b.render(jobs, max_paths=max_paths, streams=streams, graph=False)
+        a, sa = b.render(jobs, max_paths=max_paths, streams=streams, shared=shared)
+        e, se = b.render(jobs, max_paths=max_paths, streams=streams, graph=False, shared=shared)
     assert sa.batches == se.batches
     if max_paths == 6000:
-        assert sa.batches > 3 * len(jobs)
+        assert sa.batches > 3 * len(jobs) if not shared else sa.batches > 40
+    if shared and max_paths == 1 << 20:
+        assert sa.batches == 1
     for x, y, (want, _) in zip(a, e, ref):
         assert np.array_equal(x, want) and np.array_equal(y, want)
     assert sa.rays == se.rays == sum(s.rays for _, s in ref)
 
 
-def test_device_output_and_stage_by_stage(singles):
+@pytest.mark.parametrize("shared", [True, False])
+def test_device_output_and_stage_by_stage(singles, shared):
     sc, jobs, ref = singles
     n = sum(j.width * j.height for j in jobs)
     buf = pt.DeviceBuffer(12 * n)
     with pt.Batch(sc) as b:
-        _, sd = b.render(jobs, device_out=buf)
+        _, sd = b.render(jobs, device_out=buf, shared=shared)
         flat = buf.download(np.zeros(3 * n, np.float32))
-        nt, _ = b.render(jobs, tail=False)
+        nt, _ = b.render(jobs, tail=False, shared=shared)
     at = 0
     for j, x, (want, _) in zip(jobs, nt, ref):
         k = j.width * j.height
@@ -110,10 +118,15 @@ def test_device_output_and_stage_by_stage(singles):
 def test_launch_shapes_change_nothing(singles):
     sc, jobs, ref = singles
     with pt.Batch(sc) as b:
-        for S, per_cu in ((16, 0), (32, 2), (8, 3)):
-            imgs, _ = b.render(jobs, stack_entries=S, blocks_per_cu=per_cu)
+        for S, per_cu, shared in ((16, 0, True), (32, 2, True), (8, 3, True), (16, 0, False), (8, 3, False)):
+            imgs, _ = b.render(jobs, stack_entries=S, blocks_per_cu=per_cu, shared=shared)
             for x, (want, _) in zip(imgs, ref):
-                assert np.array_equal(x, want), (S, per_cu)
+                assert np.array_equal(x, want), (S, per_cu, shared)
+        # the two modes one after the other on the same batch (each keeps its own captured graph)
+        for shared in (True, False, True):
+            imgs, _ = b.render(jobs, shared=shared)
+            for x, (want, _) in zip(imgs, ref):
+                assert np.array_equal(x, want), shared
 
 
 def test_batch_job_matches_oracle():
