@@ -249,7 +249,7 @@ void render_pool(pt_batch* b, const pt_batch_job* jobs, int n_jobs, const pt_bat
     for (size_t t = 0; t < passes.size(); ++t) {
         tab_off[t] = tab_bytes;
         const size_t n = passes[t].size();
-        tab_bytes += n * sizeof(ptd::PoolSlice) + ((2 * (n + 1) * 4 + 127) & ~size_t(127));
+        tab_bytes += n * sizeof(ptd::PoolSlice) + ((3 * (n + 1) * 4 + 127) & ~size_t(127));
     }
     std::vector<unsigned char> tab(tab_bytes, 0);
     const uint32_t n_nodes = static_cast<uint32_t>(s->node_map.size());
@@ -260,7 +260,8 @@ void render_pool(pt_batch* b, const pt_batch_job* jobs, int n_jobs, const pt_bat
         auto* sl = reinterpret_cast<ptd::PoolSlice*>(tab.data() + tab_off[t]);
         auto* spath = reinterpret_cast<uint32_t*>(tab.data() + tab_off[t] + n * sizeof(ptd::PoolSlice));
         uint32_t* spix = spath + n + 1;
-        uint64_t path = 0, pix = 0;
+        uint32_t* stile = spix + n + 1;
+        uint64_t path = 0, pix = 0, tiles = 0;
         for (size_t k = 0; k < n; ++k) {
             const slice_plan& q = passes[t][k];
             const pt_batch_job& J = jobs[q.job];
@@ -280,24 +281,31 @@ void render_pool(pt_batch* b, const pt_batch_job* jobs, int n_jobs, const pt_bat
             o.n_lights = v.n_lights;
             o.flags = v.flags;
             o.pix_off = static_cast<uint32_t>(jp[q.job].pix_off);
+            const uint64_t npix = jp[q.job].npix;
+            o.tile_pix = static_cast<uint32_t>(
+                uint64_t(q.ns) <= ptd::POOL_TILE ? std::min<uint64_t>(ptd::POOL_TILE / q.ns, npix) : 1);
             spath[k] = static_cast<uint32_t>(path);
             spix[k] = static_cast<uint32_t>(pix);
+            stile[k] = static_cast<uint32_t>(tiles);
             // What the kernels rely on, checked before anything is launched:
             // every index a slice leads to lies in its array.
             const bool ok = q.ns >= 1 && J.width >= 2 && J.height >= 2 &&
                             (o.root == NONE || o.root < n_nodes) &&
                             uint64_t(o.large_off) + o.n_large <= s->view.n_large &&
                             uint64_t(o.lights_off) + o.n_lights <= s->view.n_lights &&
-                            uint64_t(o.pix_off) + jp[q.job].npix <= total_pix;
+                            uint64_t(o.pix_off) + jp[q.job].npix <= total_pix && o.tile_pix >= 1 &&
+                            (uint64_t(o.tile_pix) * o.ns <= ptd::POOL_TILE || o.tile_pix == 1);
             if (!ok) throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: internal error in a slice table");
-            path += uint64_t(jp[q.job].npix) * uint64_t(q.ns);
-            pix += jp[q.job].npix;
+            path += npix * uint64_t(q.ns);
+            pix += npix;
+            tiles += (npix + o.tile_pix - 1) / o.tile_pix;
             max_ns = std::max(max_ns, q.ns);
         }
         if (n == 0 || path > sh.cap || path >= (uint64_t(1) << 32))
             throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: internal error in a pass plan");
         spath[n] = static_cast<uint32_t>(path);
         spix[n] = static_cast<uint32_t>(pix);
+        stile[n] = static_cast<uint32_t>(tiles);
         pass_paths[t] = static_cast<uint32_t>(path);
         pass_pix[t] = static_cast<uint32_t>(pix);
         max_paths = std::max<size_t>(max_paths, path);
@@ -370,6 +378,7 @@ void render_pool(pt_batch* b, const pt_batch_job* jobs, int n_jobs, const pt_bat
         fp.slices = reinterpret_cast<const ptd::PoolSlice*>(at);
         fp.slice_path = reinterpret_cast<const uint32_t*>(at + n * sizeof(ptd::PoolSlice));
         fp.slice_pix = fp.slice_path + n + 1;
+        fp.slice_tile = fp.slice_pix + n + 1;
         fp.n_slices = static_cast<uint32_t>(n);
         fp.P = pass_paths[t];
         fp.npix = pass_pix[t];

@@ -41,6 +41,9 @@ struct FrameConst {
     uint32_t pad;
 };
 
+// Samples (float4) one accumulation tile of a shared-queue pass holds in LDS.
+constexpr uint32_t POOL_TILE = 1024;
+
 // Shared-queue batches: one pass of the bounce loop carries the paths of
 // many jobs at once, in slices. A slice is one job's pixels for a run of its
 // samples (ns a pixel from sample index index0 on); its paths are numbered
@@ -57,7 +60,8 @@ struct PoolSlice {
     uint32_t lights_off, n_lights;  // its run of the merged light list
     uint32_t flags;         // 1 sky, 2 sample lights (the job's NO_NEE applied)
     uint32_t pix_off;       // the job's first pixel in the sums
-    uint32_t pad[3];
+    uint32_t tile_pix;      // pixels per accumulation tile: tile_pix * ns <= POOL_TILE, or 1
+    uint32_t pad[2];
 };
 static_assert(sizeof(PoolSlice) == 128, "slice records are 128 bytes");
 
@@ -107,11 +111,13 @@ struct FrameParams {
     int guide_bounces;        // diffuse vertices of bounces [0, guide_bounces) sample the guide
     float* gmis;              // guided renders: per path, the product of its guided MIS factors
     // Shared-queue passes (0 slices: a single render): n_slices slices, their
-    // first paths (slice_path[0] = 0, increasing, slice_path[n_slices] = P)
-    // and first pass pixels (slice_pix, the same shape, ending at npix).
+    // first paths (slice_path[0] = 0, increasing, slice_path[n_slices] = P),
+    // first pass pixels (slice_pix, the same shape, ending at npix) and first
+    // accumulation tiles (slice_tile, ending at the pass's tile count).
     const PoolSlice* slices;
     const uint32_t* slice_path;
     const uint32_t* slice_pix;
+    const uint32_t* slice_tile;
     uint32_t n_slices;
 };
 constexpr uint32_t GUIDE_RECORDS = 2;   // diffuse vertices per path recorded for training

@@ -1199,21 +1199,46 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams fp) {
     }
 }
 
-// Shared-queue passes: per pass pixel (a slice's pixels, numbered after the
-// earlier slices'), the slice's samples added in sample order onto its job's
-// pixel sum; a job's slices come in sample order, pass after pass, so every
-// pixel sums its samples in the order a single render does.
+// Shared-queue passes: each pass pixel's slice samples added in sample order
+// onto its job's pixel sum; a job's slices come in sample order, pass after
+// pass, so every pixel sums its samples in the order a single render does.
+// A block takes a tile at a time: tile_pix pixels of one slice, whose
+// samples lie next to each other (a pixel's ns samples adjacent), copied into
+// LDS with coalesced loads and summed by one thread per pixel. A slice with
+// more samples a pixel than a tile holds is summed straight from HBM.
 __global__ __launch_bounds__(256) void k_accumulate_pool(FrameParams fp) {
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < fp.npix; g += gridDim.x * blockDim.x) {
-        const uint32_t k = slice_of(fp.slice_pix, fp.n_slices, g);
+    __shared__ float4 tile[POOL_TILE + POOL_TILE / 8];
+    const uint32_t n_tiles = fp.slice_tile[fp.n_slices];
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint32_t k = slice_of(fp.slice_tile, fp.n_slices, t);
         const PoolSlice& sl = fp.slices[k];
-        const uint32_t local = g - fp.slice_pix[k];
-        const float4* src = fp.pb + fp.slice_path[k] + static_cast<size_t>(local) * sl.ns;
-        float4* dst = fp.accum + sl.pix_off + local;
-        const float4 acc = *dst;
-        f3 sum = xyz(acc);
-        for (uint32_t s = 0; s < sl.ns; ++s) sum = sum + xyz(src[s]);
-        *dst = make_float4(sum.x, sum.y, sum.z, 0);
+        const uint32_t ns = sl.ns, tp = sl.tile_pix;
+        const uint32_t npix = fp.slice_pix[k + 1] - fp.slice_pix[k];
+        const uint32_t px0 = (t - fp.slice_tile[k]) * tp;
+        const uint32_t np = min(tp, npix - px0);
+        const float4* src = fp.pb + fp.slice_path[k] + static_cast<size_t>(px0) * ns;
+        float4* dst = fp.accum + sl.pix_off + px0;
+        if (ns <= POOL_TILE) {
+            // A pixel's row is padded by one float4 when ns >= 8 (then at most
+            // POOL_TILE / 8 pixels a tile), so the adders spread over the banks.
+            const uint32_t pitch = ns + (ns >= 8 ? 1u : 0u);
+            const uint32_t count = np * ns;
+            for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) tile[(i / ns) * pitch + i % ns] = src[i];
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < np; q += blockDim.x) {
+                const float4 acc = dst[q];
+                f3 sum = xyz(acc);
+                const float4* r = tile + q * pitch;
+                for (uint32_t s = 0; s < ns; ++s) sum = sum + xyz(r[s]);
+                dst[q] = make_float4(sum.x, sum.y, sum.z, 0);
+            }
+            __syncthreads();
+        } else if (threadIdx.x == 0) {   // tp is 1
+            const float4 acc = dst[0];
+            f3 sum = xyz(acc);
+            for (uint32_t s = 0; s < ns; ++s) sum = sum + xyz(src[s]);
+            dst[0] = make_float4(sum.x, sum.y, sum.z, 0);
+        }
     }
 }
 
@@ -1476,7 +1501,7 @@ static void launch_pool_loop(const FrameParams& fp, int trace_grid, int shade_gr
 hipError_t launch_pool_pass(const FrameParams& fp, int stack_entries, int trace_grid, int shade_grid,
                             hipStream_t stream) {
     if (fp.depth < 1 || fp.depth >= MAX_BOUNCES || fp.n_slices < 1 || !fp.slices || !fp.slice_path ||
-        !fp.slice_pix || fp.guide_mode != 0 || fp.want_aux || fp.sc.inst_root != NONE)
+        !fp.slice_pix || !fp.slice_tile || fp.guide_mode != 0 || fp.want_aux || fp.sc.inst_root != NONE)
         return hipErrorInvalidValue;
     switch (stack_entries) {
         case 8: launch_pool_loop<8>(fp, trace_grid, shade_grid, stream); break;
