@@ -588,6 +588,8 @@ def main():
                     help="the equal-time budget: the plain render at this many spp")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoiser leg")
     ap.add_argument("--no-bvh", action="store_true", help="skip the BVH build leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip pt_scene_autotune on the headline frame (run after its timed frames)")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (HBM traffic and VALU issue of the traversal stage)")
     ap.add_argument("--no-configs", action="store_true", help="skip the sphere-field and instancing legs")
@@ -711,6 +713,20 @@ def main():
         log(f"copy probe: {nbytes} bytes device to device")
         a.free()
         b.free()
+    autotune = None
+    if not args.no_tune:
+        # pt_scene_autotune on the same frame, last, since the scene keeps the
+        # shape it picks: the candidates' rates, and the tuned shape's frame
+        # time against the default shape's timed frames above.
+        t_t = time.perf_counter()
+        r = scene.autotune(W, H, spp, seed=1)
+        _, _, tst2 = scene.render(W, H, spp, seed=1, device_out=image)
+        autotune = {"stack_entries": r.stack_entries, "blocks_per_cu": r.blocks_per_cu, "max_paths": r.max_paths,
+                    "tuned_frame_ms": round(tst2.render_ms, 2), "default_frame_ms": round(gpu_ms / args.steps, 2),
+                    "seconds": round(time.perf_counter() - t_t, 1),
+                    "tried": [{"stack_entries": t[0], "blocks_per_cu": t[1], "max_paths": t[3],
+                               "mrays_per_s": round(t[4], 1)} for t in r.tried]}
+        log(f"autotune: {json.dumps(autotune)}")
     image.free()
     scene.close()
     batch = None if args.no_batch else batch_leg(pt, args.batch_jobs, max(1, args.steps), not args.batch_no_tune,
@@ -731,6 +747,7 @@ def main():
         "host_output_mrays_per_s": round(hst.rays / host_wall / 1e6, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": autotune,
         "batch": batch,
         "guided": guided,
         "denoise": denoised,

@@ -434,6 +434,18 @@ typedef struct {
 pt_status pt_batch_autotune(pt_batch* batch, const pt_batch_job* jobs, int n_jobs, int max_bounces,
                             pt_tune_result* result);
 
+/* The autotuner for one scene's renders: renders the frame `params` describe
+ * (as given; pass fewer samples for a quicker tune) under candidate launch
+ * shapes -- LDS stack entries (8, 16, 32) x traversal blocks per CU, then
+ * batches of a half and a quarter of the memory-sized one -- timing each by
+ * its device time, and keeps the fastest in the scene for its later renders
+ * and traces whose params leave stack_entries and samples_per_batch 0. The
+ * shape changes no bit of an image. result (optional): the chosen shape
+ * (max_paths: paths a batch may hold, 0 = memory-sized; streams: 1) and the
+ * rate of each candidate. The reference has no tuner: its CLI splits rows
+ * over hardware threads (main.cpp:116-158). */
+pt_status pt_scene_autotune(pt_scene* scene, const pt_render_params* params, pt_tune_result* result);
+
 /* ---------------------------------------------------------------------------
  * Learned path guide (the config-4 guided tracer). The reference has no path
  * guiding: these entry points have no counterpart there. The guide is an MLP

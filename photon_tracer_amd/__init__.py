@@ -193,6 +193,8 @@ def library() -> ctypes.CDLL:
     lib.pt_render_batch.argtypes = [vp, vp, ctypes.c_int, ctypes.POINTER(_BatchParams), vp,
                                     ctypes.POINTER(_RenderStats)]
     lib.pt_batch_autotune.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_TuneResult)]
+    lib.pt_scene_autotune.argtypes = [vp, vp, ctypes.POINTER(_TuneResult)]
+    lib.pt_scene_autotune.restype = ctypes.c_int
     for name in ("pt_batch_merge", "pt_batch_create", "pt_batch_destroy", "pt_render_batch", "pt_batch_autotune"):
         getattr(lib, name).restype = ctypes.c_int
     for name in ("pt_device_alloc", "pt_device_free", "pt_device_copy", "pt_device_info", "pt_generate",
@@ -440,7 +442,8 @@ class Scene:
         st = _RenderStats()
         if device_out is not None:
             # The image stays in HBM, in device_out; no aux in this mode.
-            assert device_out.nbytes >= 12 * max(nrows, 0) * width and not aux
+            if device_out.nbytes < 12 * max(nrows, 0) * width or aux:
+                raise ValueError("device_out must hold 12 bytes a pixel, and takes no aux")
             _check(library().pt_render(self._h, ctypes.byref(p), ctypes.c_void_p(device_out.ptr), None,
                                        ctypes.byref(st)))
             rgb, ax = None, None
@@ -452,6 +455,21 @@ class Scene:
         stats = RenderStats(st.rays, st.primary_rays, st.shadow_rays, st.render_ms, st.extend_ms, st.shadow_ms,
                             tuple(st.visits), st.batches, st.samples_per_batch, st.guide_ms, st.guide_queries)
         return rgb, ax, stats
+
+
+    def autotune(self, width: int, height: int, spp: int, max_bounces: int = 10, seed: int = 0,
+                 rows: tuple[int, int] | None = None, nee: bool = True, tail: bool = True,
+                 guide: bool = False) -> "TuneResult":
+        """pt_scene_autotune: times this render under candidate launch shapes (stack entries x traversal blocks
+        per CU, then smaller batches) and keeps the fastest for the scene's later renders."""
+        j0, j1 = rows if rows else (0, height)
+        p = _RenderParams(width, height, spp, max_bounces, seed, 0, j0, j1,
+                          (0 if nee else RENDER_NO_NEE) | (0 if tail else RENDER_NO_TAIL) |
+                          (RENDER_GUIDE if guide else 0), 0, 0)
+        r = _TuneResult()
+        _check(library().pt_scene_autotune(self._h, ctypes.byref(p), ctypes.byref(r)))
+        tried = [(t.stack_entries, t.blocks_per_cu, t.streams, t.max_paths, t.mrays_per_s) for t in r.tried[:r.n_tried]]
+        return TuneResult(r.stack_entries, r.blocks_per_cu, r.streams, r.max_paths, tried)
 
 
 class Progressive:

@@ -258,12 +258,13 @@ std::vector<uint32_t> device_nodes(const ptgp_check& chk, std::vector<uint32_t>&
     return map;
 }
 
-int choose_spb(int spp, size_t npix, int requested, size_t free_bytes, size_t extra_path_bytes) {
+int choose_spb(int spp, size_t npix, int requested, size_t free_bytes, size_t extra_path_bytes, uint64_t tuned) {
     if (requested > 0) {
         if (spp % requested) throw pt::status_error(PT_ERR_INVALID, "samples_per_batch must divide spp");
         return requested;
     }
-    const size_t target = std::min(kMaxPaths, free_bytes / 4 / (kPathBytes + extra_path_bytes));
+    size_t target = std::min(kMaxPaths, free_bytes / 4 / (kPathBytes + extra_path_bytes));
+    if (tuned) target = std::min<size_t>(target, tuned);
     const size_t cap = std::max<size_t>(1, target / std::max<size_t>(1, npix));
     int best = 1;
     for (int d = 1; d <= spp; ++d)
@@ -489,7 +490,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         // Per path: the guide's outputs; while training, its records, their
         // weights and the compacted samples.
         const size_t guide_bytes = (guided ? 64 + 4 : 0) + (training ? ptd::GUIDE_RECORDS * (64 + 4 + 48) : 0);
-        const int spb = choose_spb(P.spp, npix, P.samples_per_batch, free_bytes, guide_bytes);
+        const int spb = choose_spb(P.spp, npix, P.samples_per_batch, free_bytes, guide_bytes, s->tuned_paths);
         const size_t paths = npix * size_t(spb);
         if (paths >= (size_t(1) << 31)) throw pt::status_error(PT_ERR_LIMIT, "pt_render: batch too large");
         const int S = stack_entries_for(s, P.stack_entries);
@@ -699,6 +700,88 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             stats->guide_queries = v[7];
         }
     });
+}
+
+pt_status pt_scene_autotune(pt_scene* s, const pt_render_params* params, pt_tune_result* result) {
+    if (!s || !params) return pt::fail(PT_ERR_INVALID, "pt_scene_autotune: null argument");
+    if (params->flags & (PT_RENDER_TRAIN | PT_RENDER_TIMING | PT_RENDER_COUNT | PT_RENDER_DEVICE_OUTPUT))
+        return pt::fail(PT_ERR_INVALID, "pt_scene_autotune: TRAIN, TIMING, COUNT and DEVICE_OUTPUT renders are not tuned");
+    const int old_stack = s->tuned_stack, old_blocks = s->tuned_blocks;
+    const uint64_t old_paths = s->tuned_paths;
+    try {
+        pt_render_params P = *params;
+        P.stack_entries = 0;
+        P.samples_per_batch = 0;
+        const int rows = (P.row_begin == 0 && P.row_end == 0) ? P.height : P.row_end - P.row_begin;
+        std::vector<float> img(3 * size_t(std::max(0, rows)) * size_t(std::max(0, P.width)));
+        struct cand {
+            int S, per_cu;
+            uint64_t paths;
+            double mrays;
+        };
+        std::vector<cand> tried;
+        uint64_t auto_paths = 0;
+        // Each candidate renders the frame as given twice (the first captures
+        // its graph) and is timed by the second's device time.
+        auto rate = [&](int S, int per_cu, uint64_t paths) {
+            s->tuned_stack = S;
+            s->tuned_blocks = per_cu;
+            s->tuned_paths = paths;
+            pt_render_stats st{};
+            for (int k = 0; k < 2; ++k) {
+                const pt_status e = pt_render(s, &P, img.data(), nullptr, &st);
+                if (e != PT_OK) throw pt::status_error(e, pt_last_error());
+            }
+            if (!paths && !auto_paths) auto_paths = uint64_t(st.samples_per_batch) * (img.size() / 3);
+            tried.push_back({S, per_cu, paths, st.render_ms > 0 ? double(st.rays) / (st.render_ms * 1e3) : 0.0});
+        };
+        auto best = [&] {
+            return *std::max_element(tried.begin(), tried.end(), [](const cand& x, const cand& y) { return x.mrays < y.mrays; });
+        };
+        // Stage 1: LDS stack entries x traversal blocks per CU.
+        const bool inst = s->view.inst_root != NONE;
+        for (int S : {8, 16, 32}) {
+            const int occ = ptd::path_blocks_per_cu(S, inst);
+            std::vector<int> blocks{occ};
+            if (occ > 2) blocks.push_back(occ - 1);
+            if (occ > 4) blocks.push_back(occ / 2);
+            for (int pc : blocks) rate(S, pc, 0);
+        }
+        cand c = best();
+        // Stage 2: smaller batches (queue sizes) than the memory-sized one.
+        for (uint64_t div : {2, 4})
+            if (auto_paths / div >= img.size() / 3) rate(c.S, c.per_cu, auto_paths / div);
+        c = best();
+        s->tuned_stack = c.S;
+        s->tuned_blocks = c.per_cu;
+        s->tuned_paths = c.paths;
+        if (result) {
+            *result = pt_tune_result{};
+            result->stack_entries = c.S;
+            result->blocks_per_cu = c.per_cu;
+            result->streams = 1;
+            result->max_paths = c.paths;
+            result->n_tried = static_cast<int32_t>(std::min<size_t>(tried.size(), 32));
+            for (int k = 0; k < result->n_tried; ++k) {
+                result->tried[k].stack_entries = tried[k].S;
+                result->tried[k].blocks_per_cu = tried[k].per_cu;
+                result->tried[k].streams = 1;
+                result->tried[k].max_paths = tried[k].paths;
+                result->tried[k].mrays_per_s = tried[k].mrays;
+            }
+        }
+        return PT_OK;
+    } catch (const pt::status_error& e) {
+        s->tuned_stack = old_stack;
+        s->tuned_blocks = old_blocks;
+        s->tuned_paths = old_paths;
+        return pt::fail(e.code, e.what());
+    } catch (const std::exception& e) {
+        s->tuned_stack = old_stack;
+        s->tuned_blocks = old_blocks;
+        s->tuned_paths = old_paths;
+        return pt::fail(PT_ERR_INVALID, e.what());
+    }
 }
 
 pt_status pt_device_alloc(size_t bytes, void** ptr) {

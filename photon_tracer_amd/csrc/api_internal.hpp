@@ -99,6 +99,7 @@ struct pt_scene {
     std::vector<uint32_t> node_map;
     uint32_t device_node(uint32_t k) const { return k == ptd::NONE ? k : node_map[k]; }
     int tuned_stack = 0, tuned_blocks = 0;
+    uint64_t tuned_paths = 0;   // paths one render batch may hold (pt_scene_autotune; 0: memory-sized)
     ptapi::render_bufs rb;
     pt_guide* guide = nullptr;   // attached learned guide, if any
     std::vector<pt_guide*> guides;   // every guide made on this scene (they use its stream)

@@ -171,3 +171,19 @@ def test_progressive_driver_equals_one_render():
     assert steps[0] == 1 and sum(steps) == spp
     assert all(n & (n - 1) == 0 and n <= 64 for n in steps[:-1])
     assert np.allclose(img, want, rtol=2e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["crowd", "config1"])
+def test_scene_autotune_keeps_the_image(name):
+    """pt_scene_autotune times the frame under candidate launch shapes and keeps the fastest; the shape
+    changes no bit of the image, before or after tuning, and tuning a training render is refused."""
+    ptgp = scene_of(name)
+    with pt.Scene(ptgp) as s:
+        before, _, sb = s.render(160, 90, 16, seed=3)
+        r = s.autotune(160, 90, 16, seed=3)
+        after, _, sa = s.render(160, 90, 16, seed=3)
+        print(f"{name}: tuned stack {r.stack_entries}, blocks/CU {r.blocks_per_cu}, paths {r.max_paths}; "
+              f"tried {[(t[0], t[1], t[3], round(t[4])) for t in r.tried]}", flush=True)
+    assert r.stack_entries in (8, 16, 32) and r.blocks_per_cu >= 1 and r.streams == 1
+    assert len(r.tried) >= 6 and (r.stack_entries, r.blocks_per_cu, r.max_paths) in [(t[0], t[1], t[3]) for t in r.tried]
+    assert np.array_equal(before, after) and sb.rays == sa.rays
