@@ -373,13 +373,15 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
                 train_ms += st.render_ms
                 n += 1
             steps, used, loss = g.stats()
-            # The guided render's cost per spp (a 16-spp render), then the rest
-            # of the budget in whole 16-spp batches (a sample count with few
-            # divisors would split into many small batches).
-            _, _, st1 = s.render(W, H, 16, seed=7, guide=True)
-            per_spp = st1.render_ms / 16
-            spp_g = max(16, int((budget - train_ms) / per_spp) // 16 * 16)
-            guided, _, sg = s.render(W, H, spp_g, seed=21, guide=True, samples_per_batch=16)
+            # The guided render's cost per spp, measured at the batch size the
+            # plain render ran at (the library's choice for this frame), then
+            # the rest of the budget in whole batches of that size, so both
+            # tracers run the same launch shapes.
+            spb = max(16, sp.samples_per_batch)
+            _, _, st1 = s.render(W, H, spb, seed=7, guide=True)
+            per_spp = st1.render_ms / spb
+            spp_g = max(spb, int((budget - train_ms) / per_spp) // spb * spb)
+            guided, _, sg = s.render(W, H, spp_g, seed=21, guide=True, samples_per_batch=spb)
             e_guided = rel_mse(guided)
             guided_eq, _, _ = s.render(W, H, spp_plain, seed=41, guide=True)
             e_guided_eq = rel_mse(guided_eq)
@@ -388,13 +390,14 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
                 log(f"guided: network roofline {json.dumps(net)}")
             # The time to reach the plain tracer's relMSE, measured: a guided
             # render (new seed) at the sample count where relMSE ~ 1/spp
-            # predicts the match, in whole 16-spp batches, and 25% more
-            # samples at a time (at most three times) until its relMSE is at
-            # most plain's. Training time counts.
+            # predicts the match, a multiple of 16 spp in batches of the
+            # library's choice (as the plain render's), and 25% more samples
+            # at a time (at most three times) until its relMSE is at most
+            # plain's. Training time counts.
             t_pred = train_ms + sg.render_ms * e_guided / e_plain
             spp_m = max(16, math.ceil(spp_g * e_guided / e_plain / 16) * 16)
             for attempt in range(4):
-                gm, _, sm = s.render(W, H, spp_m, seed=61 + attempt, guide=True, samples_per_batch=16)
+                gm, _, sm = s.render(W, H, spp_m, seed=61 + attempt, guide=True)
                 e_m = rel_mse(gm)
                 if e_m <= e_plain or attempt == 3:
                     break
