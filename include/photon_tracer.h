@@ -157,8 +157,8 @@ pt_status pt_pack_ex(const uint32_t* ptsd, size_t n_words, int flags, uint32_t**
  * flags: PT_BVH_DEVICE builds on the GPU (level-parallel binning and
  * partition, one wave per small subtree, parallel collapse) with the same
  * result word for word; 0 builds on host threads. ms: wall-clock of the
- * build, the device's transfers and allocations included, the copies into
- * the returned arrays not (may be NULL). At most 2^30 boxes (2^28 for the
+ * whole call, the device's transfers and allocations and the copies into the
+ * returned arrays included (may be NULL). At most 2^30 boxes (2^28 for the
  * device build, PT_ERR_LIMIT); the device build refuses boxes with a NaN
  * coordinate (PT_ERR_INVALID) and hands boxes with a -0.0 coordinate to the
  * host builder (the reference keeps whichever of -0.0 and +0.0 comes first;

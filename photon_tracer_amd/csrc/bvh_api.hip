@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "api_internal.hpp"
 #include "bvh_gpu.hpp"
@@ -21,57 +23,102 @@ constexpr size_t kMaxBoxes = size_t(1) << 30;
 
 // Device buffers kept between builds (grown, never shrunk, until the
 // process ends): allocating a few hundred MB per call cost more than the
-// build of 1M boxes itself. One build at a time per process.
+// build of 1M boxes itself. The pinned host buffer stages the tree on its
+// way back (a copy from pageable memory runs at half the link's rate). One
+// build at a time per process.
 struct Pool {
     std::mutex m;
     dbuf boxes, order, nodes, scratch;
+    void* host = nullptr;   // pinned, host_bytes long
+    size_t host_bytes = 0;
+    void ensure_host(size_t n) {
+        if (n <= host_bytes && host) return;
+        if (host) (void)hipHostFree(host);
+        host = nullptr;
+        host_bytes = 0;
+        HIPC(hipHostMalloc(&host, n, hipHostMallocDefault));
+        host_bytes = n;
+    }
 };
 Pool& pool() {
     static Pool* p = new Pool;   // never destroyed: no device frees after the runtime is gone
     return *p;
 }
 
-}  // namespace
+// memcpy split over host threads: the destinations here are fresh
+// allocations, whose first touch (a page fault per 4 KB) costs more than
+// the copy itself; spread over threads, the faults are taken in parallel.
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kPiece = size_t(4) << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t pieces = std::min<size_t>(std::min(hw, 16u), bytes / kPiece);
+    if (pieces < 2) {
+        if (bytes) std::memcpy(dst, src, bytes);
+        return;
+    }
+    auto part = [&](size_t k) {
+        const size_t lo = bytes * k / pieces, hi = bytes * (k + 1) / pieces;
+        std::memcpy(static_cast<char*>(dst) + lo, static_cast<const char*>(src) + lo, hi - lo);
+    };
+    std::vector<std::thread> th;
+    th.reserve(pieces - 1);
+    for (size_t k = 1; k < pieces; ++k) th.emplace_back(part, k);
+    part(0);
+    for (std::thread& t : th) t.join();
+}
 
-// build_bvh's tree, built by the device builder.
-pt::bvh_tree device_bvh(const std::vector<pt::box3>& boxes) {
-    static_assert(sizeof(pt::box3) == 24, "boxes are 6 floats");
-    pt::bvh_tree out;
-    const size_t n = boxes.size();
-    if (n == 0) return out;
+// The device build over n caller boxes (host memory, 6 floats each): on
+// success the tree is in the pool, *k nodes in P.nodes and the order in
+// P.order, and its words are staged in P.host (nodes, then order). Returns
+// false for boxes with a -0.0 coordinate, which the host builder takes: -0
+// and +0 compare equal, so std::min and std::max keep whichever of the two
+// comes first in the reference's order, while the device reductions (atomics
+// on order-preserving keys, shuffles) always pick one sign. With a NaN,
+// std::min's result depends on the order it is taken in, so the parallel
+// build could not promise the host's tree: refused. The boxes are checked
+// on the device as the build starts (one pass over them), not on the host.
+bool device_build(Pool& P, const float* boxes, size_t n, uint32_t* k) {
     if (n > ptd::bvhg::kMaxDeviceBoxes)
         throw pt::status_error(PT_ERR_LIMIT, "device BVH build: more than 2^28 primitives");
-    // With a NaN, std::min's result depends on the order it is taken in, so
-    // the parallel build could not promise the host's tree: refused.
-    bool negative_zero = false;
-    for (const pt::box3& b : boxes)
-        for (int a = 0; a < 3; ++a) {
-            if (std::isnan(b.lo[a]) || std::isnan(b.hi[a]))
-                throw pt::status_error(PT_ERR_INVALID, "device BVH build: a box has a NaN coordinate");
-            negative_zero |= (b.lo[a] == 0 && std::signbit(b.lo[a])) || (b.hi[a] == 0 && std::signbit(b.hi[a]));
-        }
-    // -0 and +0 compare equal, so std::min and std::max keep whichever of the
-    // two comes first in the reference's order, while the device reductions
-    // (atomics on order-preserving keys, shuffles) always pick one sign. The
-    // node words could then differ in a sign bit: such boxes are built by the
-    // host builder, which takes the reference's order.
-    if (negative_zero) return pt::build_bvh(boxes);
-    Pool& P = pool();
-    std::lock_guard<std::mutex> lock(P.m);
     P.boxes.ensure(24 * n);
     P.order.ensure(4 * n);
     P.nodes.ensure(sizeof(pt::bvh_node) * n);
     const size_t scratch = ptd::bvhg::scratch_bytes(static_cast<uint32_t>(n));
     P.scratch.ensure(scratch);
-    HIPC(hipMemcpy(P.boxes.p, boxes.data(), 24 * n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(P.boxes.p, boxes, 24 * n, hipMemcpyHostToDevice));
+    uint32_t flags = 0;
+    *k = 0;
+    HIPC(ptd::bvhg::build(P.boxes.as<float>(), static_cast<uint32_t>(n), P.order.as<uint32_t>(), P.nodes.p, k,
+                          &flags, P.scratch.p, scratch, nullptr));
+    if (flags & ptd::bvhg::BOX_NAN)
+        throw pt::status_error(PT_ERR_INVALID, "device BVH build: a box has a NaN coordinate");
+    if (flags & ptd::bvhg::BOX_NEG_ZERO) return false;
+    if (*k == 0 || *k > n) throw pt::status_error(PT_ERR_DEVICE, "BVH build: bad node count from the device");
+    const size_t node_bytes = sizeof(pt::bvh_node) * *k;
+    P.ensure_host(node_bytes + 4 * n);
+    HIPC(hipMemcpyAsync(P.host, P.nodes.p, node_bytes, hipMemcpyDeviceToHost, nullptr));
+    HIPC(hipMemcpyAsync(static_cast<char*>(P.host) + node_bytes, P.order.p, 4 * n, hipMemcpyDeviceToHost, nullptr));
+    HIPC(hipStreamSynchronize(nullptr));
+    return true;
+}
+
+}  // namespace
+
+// build_bvh's tree, built by the device builder (the host builder for boxes
+// with a -0.0 coordinate).
+pt::bvh_tree device_bvh(const std::vector<pt::box3>& boxes) {
+    static_assert(sizeof(pt::box3) == 24, "boxes are 6 floats");
+    pt::bvh_tree out;
+    const size_t n = boxes.size();
+    if (n == 0) return out;
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lock(P.m);
     uint32_t k = 0;
-    HIPC(ptd::bvhg::build(P.boxes.as<float>(), static_cast<uint32_t>(n), P.order.as<uint32_t>(), P.nodes.p, &k,
-                          P.scratch.p, scratch, nullptr));
-    if (k == 0 || k > n) throw pt::status_error(PT_ERR_DEVICE, "BVH build: bad node count from the device");
-    out.nodes.resize(k);
-    out.order.resize(n);
-    HIPC(hipMemcpy(out.nodes.data(), P.nodes.p, sizeof(pt::bvh_node) * k, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(out.order.data(), P.order.p, 4 * n, hipMemcpyDeviceToHost));
+    if (!device_build(P, reinterpret_cast<const float*>(boxes.data()), n, &k)) return pt::build_bvh(boxes);
+    const pt::bvh_node* nd = static_cast<const pt::bvh_node*>(P.host);
+    const uint32_t* ord = reinterpret_cast<const uint32_t*>(nd + k);
+    out.nodes.assign(nd, nd + k);
+    out.order.assign(ord, ord + n);
     return out;
 }
 
@@ -89,24 +136,51 @@ pt_status pt_bvh_build(const float* boxes, size_t n, int flags, uint32_t** nodes
     if (flags & ~PT_BVH_DEVICE) return pt::fail(PT_ERR_INVALID, "pt_bvh_build: unknown flags");
     if (n > ptapi::kMaxBoxes) return pt::fail(PT_ERR_LIMIT, "pt_bvh_build: more than 2^30 boxes");
     PT_GUARD({
-        std::vector<pt::box3> b(n);
-        if (n) std::memcpy(b.data(), boxes, 24 * n);
         const auto t0 = std::chrono::steady_clock::now();
-        const pt::bvh_tree t = (flags & PT_BVH_DEVICE) ? ptapi::device_bvh(b) : pt::build_bvh(b);
-        const auto t1 = std::chrono::steady_clock::now();
-        size_t n_order = 0;
-        uint32_t* words = static_cast<uint32_t*>(std::malloc(sizeof(pt::bvh_node) * t.nodes.size() + 4));
-        if (!words) throw std::bad_alloc();
-        if (!t.nodes.empty()) std::memcpy(words, t.nodes.data(), sizeof(pt::bvh_node) * t.nodes.size());
-        try {
-            *order = pt::copy_out(t.order, &n_order);
-        } catch (...) {
-            std::free(words);
-            throw;
+        uint32_t* words = nullptr;
+        size_t n_nodes_out = 0;
+        bool done = false;
+        if ((flags & PT_BVH_DEVICE) && n > 0) {
+            // Straight from the caller's boxes to the device, and from the
+            // pinned staging copy into the returned arrays.
+            ptapi::Pool& P = ptapi::pool();
+            std::lock_guard<std::mutex> lock(P.m);
+            uint32_t k = 0;
+            if (ptapi::device_build(P, boxes, n, &k)) {
+                const size_t node_bytes = sizeof(pt::bvh_node) * k;
+                words = static_cast<uint32_t*>(std::malloc(node_bytes + 4));
+                uint32_t* ord = static_cast<uint32_t*>(std::malloc(4 * n + 4));
+                if (!words || !ord) {
+                    std::free(words);
+                    std::free(ord);
+                    throw std::bad_alloc();
+                }
+                ptapi::parallel_copy(words, P.host, node_bytes);
+                ptapi::parallel_copy(ord, static_cast<const char*>(P.host) + node_bytes, 4 * n);
+                *order = ord;
+                n_nodes_out = k;
+                done = true;
+            }
+        }
+        if (!done) {
+            std::vector<pt::box3> b(n);
+            if (n) std::memcpy(b.data(), boxes, 24 * n);
+            const pt::bvh_tree t = pt::build_bvh(b);
+            size_t n_order = 0;
+            words = static_cast<uint32_t*>(std::malloc(sizeof(pt::bvh_node) * t.nodes.size() + 4));
+            if (!words) throw std::bad_alloc();
+            if (!t.nodes.empty()) std::memcpy(words, t.nodes.data(), sizeof(pt::bvh_node) * t.nodes.size());
+            try {
+                *order = pt::copy_out(t.order, &n_order);
+            } catch (...) {
+                std::free(words);
+                throw;
+            }
+            n_nodes_out = t.nodes.size();
         }
         *nodes = words;
-        *n_nodes = t.nodes.size();
-        if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        *n_nodes = n_nodes_out;
+        if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     });
 }
 

@@ -231,7 +231,10 @@ __device__ __forceinline__ uint32_t bin_init_word(int i) {
 }
 
 // ---------------------------------------------------------------------------
-// Set-up: centroids (aabb::center, aabb.hpp:22), identity order, counters.
+// Set-up: centroids (aabb::center, aabb.hpp:22), identity order, counters,
+// and the boxes' flags in counters[2] (zeroed before the launch): BOX_NAN
+// if a coordinate is NaN, BOX_NEG_ZERO if one is -0.0, one atomic per wave
+// that found either.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_setup(const float* __restrict__ boxes, uint32_t n, float4* centers,
                                                    uint32_t* order, uint32_t* counters, uint32_t* depth_of,
@@ -242,11 +245,22 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float* __restrict__ boxe
         counters[0] = 1;   // node 0 is the root
         counters[1] = 0;   // deepest binary depth
     }
-    if (i >= n) return;
-    float lo[3], hi[3];
-    load_box(boxes, i, lo, hi);
-    centers[i] = make_float4((lo[0] + hi[0]) * 0.5f, (lo[1] + hi[1]) * 0.5f, (lo[2] + hi[2]) * 0.5f, 0.f);
-    order[i] = i;
+    uint32_t flags = 0;
+    if (i < n) {
+        float lo[3], hi[3];
+        load_box(boxes, i, lo, hi);
+        centers[i] = make_float4((lo[0] + hi[0]) * 0.5f, (lo[1] + hi[1]) * 0.5f, (lo[2] + hi[2]) * 0.5f, 0.f);
+        order[i] = i;
+        for (int a = 0; a < 3; ++a) {
+            flags |= (__builtin_isnan(lo[a]) || __builtin_isnan(hi[a])) ? BOX_NAN : 0u;
+            flags |= (__float_as_uint(lo[a]) == 0x80000000u || __float_as_uint(hi[a]) == 0x80000000u) ? BOX_NEG_ZERO
+                                                                                                     : 0u;
+        }
+    }
+    const unsigned long long nan_lanes = __ballot((flags & BOX_NAN) != 0);
+    const unsigned long long zero_lanes = __ballot((flags & BOX_NEG_ZERO) != 0);
+    if ((threadIdx.x & 63) == 0 && (nan_lanes | zero_lanes))
+        atomicOr(&counters[2], (nan_lanes ? BOX_NAN : 0u) | (zero_lanes ? BOX_NEG_ZERO : 0u));
 }
 
 // ---------------------------------------------------------------------------
@@ -915,8 +929,9 @@ size_t scratch_bytes(uint32_t n) {
 }
 
 hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_nodes, uint32_t* n_nodes,
-                 void* d_scratch, size_t scratch_size, hipStream_t st) {
-    if (n == 0 || n > kMaxDeviceBoxes || !d_scratch || scratch_size < scratch_bytes(n)) return hipErrorInvalidValue;
+                 uint32_t* box_flags, void* d_scratch, size_t scratch_size, hipStream_t st) {
+    if (n == 0 || n > kMaxDeviceBoxes || !d_scratch || scratch_size < scratch_bytes(n) || !box_flags)
+        return hipErrorInvalidValue;
     const uint32_t cap = 4 * n;   // binary node slots, as scratch_bytes counts them
     const uint32_t max_segs = n / kSmall + 2;
     const uint32_t max_chunks = n / kChunk + max_segs + 2;
@@ -946,8 +961,16 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
     SmallSeg* d_small;   // small segments are disjoint and non-empty: at most n
     BVH_TRY(mem.get(&d_small, n));
 
+    BVH_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), st));
     k_setup<<<blocks(n), kBlock, 0, st>>>(d_boxes, n, centers, d_order, counters, depth_of, cap);
     BVH_TRY(hipGetLastError());
+    // Boxes with a NaN or a -0.0 coordinate stop here (the caller decides).
+    uint32_t flags = 0;
+    BVH_TRY(hipMemcpyAsync(&flags, counters + 2, 4, hipMemcpyDeviceToHost, st));
+    BVH_TRY(hipStreamSynchronize(st));
+    *box_flags = flags;
+    *n_nodes = 0;
+    if (flags) return hipSuccess;
 
     std::vector<SmallSeg> small;
     uint32_t level_nodes = 1;   // the root, then two per split of a level pass
