@@ -73,14 +73,54 @@ __device__ __forceinline__ float blob(float v, int bins, int i) {
     return blob_scaled(v * (static_cast<float>(bins) * BLOB_K), (static_cast<float>(i) + 0.5f) * BLOB_K);
 }
 
+// The normal's angles for the encoding, from polynomial approximations of
+// Abramowitz and Stegun's Handbook of Mathematical Functions (Hastings'
+// fits): arccos by 4.4.46 (|error| <= 2e-8 on [0, 1], reflected for x < 0)
+// and arctan by 4.4.49 (|error| <= 2e-8 on [0, 1]) after folding the
+// quadrant into [0, 1]. Their error, with FP32 evaluation and the hardware
+// reciprocal a few 1e-7, is far below the FP16 rounding of the one-blob
+// inputs they feed; training and inference share them. atan2 keeps IEEE's
+// signed-zero cases: atan2(+-0, -0) = +-pi, atan2(+-0, +0) = +-0.
+__device__ __forceinline__ float enc_acos(float x) {
+    const float a = fabsf(x);
+    float p = -0.0012624911f;
+    p = p * a + 0.0066700901f;
+    p = p * a - 0.0170881256f;
+    p = p * a + 0.0308918810f;
+    p = p * a - 0.0501743046f;
+    p = p * a + 0.0889789874f;
+    p = p * a - 0.2145988016f;
+    p = p * a + 1.5707963050f;
+    const float r = __builtin_amdgcn_sqrtf(1.0f - a) * p;
+    return x < 0 ? PI_F - r : r;
+}
+__device__ __forceinline__ float enc_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float hi = fmaxf(ax, ay), lo = fminf(ax, ay);
+    const float t = hi > 0 ? lo * __builtin_amdgcn_rcpf(hi) : 0.0f;
+    const float t2 = t * t;
+    float p = 0.0028662257f;
+    p = p * t2 - 0.0161657367f;
+    p = p * t2 + 0.0429096138f;
+    p = p * t2 - 0.0752896400f;
+    p = p * t2 + 0.1065626393f;
+    p = p * t2 - 0.1420889944f;
+    p = p * t2 + 0.1999355085f;
+    p = p * t2 - 0.3333314528f;
+    float r = t + t * (p * t2);      // arctan(t), t in [0, 1]
+    if (ay > ax) r = 0.5f * PI_F - r;
+    if (__builtin_signbit(x)) r = PI_F - r;
+    return copysignf(r, y);
+}
+
 // The five values a shading point is encoded from: its position normalised
 // to the box per axis, and the normal's polar angle / pi and azimuth / (2 pi).
 __device__ __forceinline__ void encode_scalars(const GuideView& g, f3 x, f3 n, float e[5]) {
     e[0] = clamp01((x.x - g.lo.x) * g.inv_ext.x);
     e[1] = clamp01((x.y - g.lo.y) * g.inv_ext.y);
     e[2] = clamp01((x.z - g.lo.z) * g.inv_ext.z);
-    e[3] = clamp01(acosf(fminf(fmaxf(n.z, -1.0f), 1.0f)) * (1 / PI_F));
-    e[4] = clamp01((atan2f(n.y, n.x) + PI_F) * (1 / (2 * PI_F)));
+    e[3] = clamp01(enc_acos(fminf(fmaxf(n.z, -1.0f), 1.0f)) * (1 / PI_F));
+    e[4] = clamp01((enc_atan2(n.y, n.x) + PI_F) * (1 / (2 * PI_F)));
 }
 
 // The 64 inputs of a shading point, as one row: 16 bins per position axis,
@@ -252,10 +292,11 @@ __device__ __forceinline__ f3 mixture_sample(const T* o, float u0, float u1, flo
 // estimator stays unbiased.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void decode_lobe(float o0, float o1, float o2, float& x, float& y, float& s) {
-    const float th = PI_F * fast_sigmoid(o0), ph = 2 * PI_F * fast_sigmoid(o1);
-    float st, ct, sp, cp;
-    __sincosf(th, &st, &ct);
-    __sincosf(ph, &sp, &cp);
+    // The angles in turns (theta = pi sigmoid, phi = 2 pi sigmoid), the unit
+    // the hardware's sine and cosine take.
+    const float th = 0.5f * fast_sigmoid(o0), ph = fast_sigmoid(o1);
+    const float st = __builtin_amdgcn_sinf(th), ct = __builtin_amdgcn_cosf(th);
+    const float sp = __builtin_amdgcn_sinf(ph), cp = __builtin_amdgcn_cosf(ph);
     x = st * cp;
     y = st * sp;
     s = copysignf(__expf(fminf(fmaxf(o2, LOG_KAPPA_MIN), LOG_KAPPA_MAX)), ct);
@@ -500,12 +541,10 @@ typedef h8 RegActs[2][4];
 // The encoded inputs in that layout (the first layer's chunks hold columns in
 // their natural order: chunk kb, half kh = inputs 16 kb + 8 kh + j). Same
 // arithmetic as encode(), so the same FP16 inputs.
-__device__ __forceinline__ void encode_regs(const GuideView& g, f3 x, f3 n, RegActs& X) {
+// From the five encoding scalars of the lane's two queries: q0 of query c,
+// q1 of query 32 + c.
+__device__ __forceinline__ void encode_blobs(const float (&q0)[5], const float (&q1)[5], RegActs& X) {
     const int kh = lane_kh();
-    float e[5], o[5];
-    encode_scalars(g, x, n, e);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) o[v] = __shfl_xor(e[v], 32, 64);   // the query of the other half's lane
     float c16[8], c8[8];   // bin centres of this lane's half, scaled
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -514,9 +553,7 @@ __device__ __forceinline__ void encode_regs(const GuideView& g, f3 x, f3 n, RegA
     }
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
-        float q[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) q[v] = nb == kh ? e[v] : o[v];
+        const float(&q)[5] = nb == 0 ? q0 : q1;
 #pragma unroll
         for (int kb = 0; kb < 3; ++kb) {
             const float vk = q[kb] * (16.0f * BLOB_K);
@@ -527,6 +564,21 @@ __device__ __forceinline__ void encode_regs(const GuideView& g, f3 x, f3 n, RegA
 #pragma unroll
         for (int j = 0; j < 8; ++j) X[nb][3][j] = static_cast<_Float16>(blob_scaled(vk, c8[j]));
     }
+}
+
+// The lane's own query (32 kh + c) at point x with normal n: its scalars,
+// and the partner lane's (the other half's) for the lane's other query.
+__device__ __forceinline__ void encode_regs(const GuideView& g, f3 x, f3 n, RegActs& X) {
+    float e[5], q0[5], q1[5];
+    encode_scalars(g, x, n, e);
+    const bool upper = lane_kh() != 0;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const float o = __shfl_xor(e[v], 32, 64);
+        q0[v] = upper ? o : e[v];
+        q1[v] = upper ? e[v] : o;
+    }
+    encode_blobs(q0, q1, X);
 }
 
 // ReLU of 8 values rounded to FP16: the conversion (round to nearest even),

@@ -532,6 +532,14 @@ struct Surface {
     float tu, tv;   // texture coordinates (triangle.hpp:72-73; 0 on spheres, hittable.hpp:14)
 };
 
+// A unit vector from the hardware's reciprocal square root (within a few
+// ulp of the divisions normalize() takes): for the guide network's inputs,
+// which are rounded to FP16 after the encoding.
+__device__ __forceinline__ f3 normalize_fast(f3 a) { return a * __builtin_amdgcn_rsqf(dot(a, a)); }
+
+// FAST: the shading normal from normalize_fast (the guide's inference, which
+// needs only the point, the normal and the material).
+template <bool FAST = false>
 __device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& h, f3 o, f3 d, bool nee) {
     const SceneView& sc = fp.sc;
     Surface sf;
@@ -557,7 +565,7 @@ __device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& 
         mat = __float_as_uint(tr[0].w);
         const f3 n0 = xyz(tr[3]), n1 = xyz(tr[4]), n2 = xyz(tr[5]), fn = xyz(tr[6]);
         const float b0 = 1 - h.u - h.v;
-        const f3 sn = normalize(n0 * b0 + n1 * h.u + n2 * h.v);
+        const f3 sn = FAST ? normalize_fast(n0 * b0 + n1 * h.u + n2 * h.v) : normalize(n0 * b0 + n1 * h.u + n2 * h.v);
         // uv in the rows' w words: (e1, u0) (e2, v0) (n0, u1) (n1, v1) (n2, u2) (fn, v2)
         sf.tu = tr[1].w * b0 + tr[3].w * h.u + tr[5].w * h.v;
         sf.tv = tr[2].w * b0 + tr[4].w * h.u + tr[6].w * h.v;
@@ -572,8 +580,9 @@ __device__ __forceinline__ Surface surface_at(const FrameParams& fp, const Hit& 
             front = dot(ld, fn) < 0;
             const f3 ln = front ? sn : -sn;
             // the transpose of to_object (transform.hpp:23-27)
-            nrm = normalize(mk(r0.x * ln.x + r1.x * ln.y + r2.x * ln.z, r0.y * ln.x + r1.y * ln.y + r2.y * ln.z,
-                               r0.z * ln.x + r1.z * ln.y + r2.z * ln.z));
+            const f3 wn = mk(r0.x * ln.x + r1.x * ln.y + r2.x * ln.z, r0.y * ln.x + r1.y * ln.y + r2.y * ln.z,
+                             r0.z * ln.x + r1.z * ln.y + r2.z * ln.z);
+            nrm = FAST ? normalize_fast(wn) : normalize(wn);
             const uint32_t im = __float_as_uint(in[6].y);
             if (im != NONE) mat = im;
         }
@@ -780,11 +789,72 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
 // on a diffuse (Lambertian) surface, the network at its point and normal,
 // decoded to 32 FP16 values at gout[slot] (per lobe: axis x, y, kappa with
 // the axis' z sign, weight). The block first copies the network's weights
-// into LDS; then each wave takes 64 slots at a time and keeps them in
-// registers from the encoding to the outputs (no barrier after the copy);
-// waves with no diffuse hit skip the network. Weights read from the caches
-// instead of LDS made the kernel 2.5x slower: their per-layer addresses,
-// hoisted out of the loop by the compiler, cost half the registers.
+// into LDS. Each wave then walks its runs of 64 slots and stages the
+// diffuse hits among them (their five encoding scalars and slot) in its own
+// LDS list; whenever 64 are staged the network runs on them, in registers
+// from the encoding to the outputs, so the matrix unit sees full batches
+// whatever the share of metal, glass and lights among the hits; the last
+// partial batch runs at the end. A query's outputs do not depend on which
+// others share its batch. Weights read from the caches instead of LDS made
+// the kernel 2.5x slower: their per-layer addresses, hoisted out of the loop
+// by the compiler, cost half the registers.
+struct alignas(16) InferEntry {
+    float e[5];
+    uint32_t slot;
+    uint32_t pad[2];
+};
+constexpr int INFER_WAVES = INFER_BLOCK / 64;
+
+// The network and decoding for staged entries [0, count) of a wave's list
+// (count <= 64): lane (c, kh) encodes entries c and 32 + c and writes their
+// lobes 4 kh .. 4 kh + 3.
+__device__ __forceinline__ void infer_staged(const FrameParams& fp, const guide::LdsNet& net, const InferEntry* st,
+                                             uint32_t count) {
+    const int c = guide::lane_c(), kh = guide::lane_kh();
+    float q0[5], q1[5];
+    const InferEntry& a = st[c];
+    const InferEntry& bq = st[32 + c];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        q0[v] = a.e[v];
+        q1[v] = bq.e[v];
+    }
+    const uint32_t slot[2] = {a.slot, bq.slot};
+    guide::RegActs X;
+    guide::encode_blobs(q0, q1, X);
+    guide::f16v O[2];
+    guide::forward_regs(guide::LdsWeights{net}, X, O);
+    // The lane holds whole lobes (2 g4 + kh, g4 < 4) of its two entries: each
+    // is decoded once here for the shade stage, the softmax taking the other
+    // half's four lobes from the partner lane.
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+        float mx = O[nb][3];
+#pragma unroll
+        for (int g4 = 1; g4 < 4; ++g4) mx = fmaxf(mx, O[nb][4 * g4 + 3]);
+        mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+        float e[4], sum = 0;
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4) {
+            e[g4] = __expf(O[nb][4 * g4 + 3] - mx);
+            sum += e[g4];
+        }
+        sum += __shfl_xor(sum, 32, 64);
+        const float inv = __fdividef(1.0f, sum);
+        if (static_cast<uint32_t>(32 * nb + c) < count) {
+            _Float16* out = fp.gout + static_cast<size_t>(slot[nb]) * guide::OUT + 4 * kh;
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                float x, y, sk;
+                guide::decode_lobe(O[nb][4 * g4], O[nb][4 * g4 + 1], O[nb][4 * g4 + 2], x, y, sk);
+                const guide::h4 hv{static_cast<_Float16>(x), static_cast<_Float16>(y), static_cast<_Float16>(sk),
+                                   static_cast<_Float16>(e[g4] * inv)};
+                *reinterpret_cast<guide::h4*>(out + 8 * g4) = hv;
+            }
+        }
+    }
+}
+
 template <bool PRIMARY>
 __global__ __launch_bounds__(INFER_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_guide_infer(FrameParams fp, int b) {
     if (fp.frame->tail_from <= static_cast<uint32_t>(b)) return;
@@ -793,69 +863,73 @@ __global__ __launch_bounds__(INFER_BLOCK) __attribute__((amdgpu_waves_per_eu(4))
     if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1;
     __shared__ guide::LdsNet net;
+    __shared__ InferEntry stage[INFER_WAVES][128];
     guide::load_net(fp.gv, net);
-    const int c = guide::lane_c(), kh = guide::lane_kh();
+    InferEntry* st = stage[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t stride = gridDim.x * INFER_BLOCK;
-    for (uint32_t base = blockIdx.x * INFER_BLOCK + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint32_t q = base + lane;
-        bool diffuse = false;
-        f3 pnt = mk(0, 0, 0), nrm = mk(0, 0, 1);
-        if (q < n) {
-            const Hit h = fp.hits[q];
-            if (h.kind != 0) {
-                f3 o, d;
-                if (PRIMARY) {
-                    const CameraSample cs = camera_sample(fp, fc, q);
-                    o = cs.o;
-                    d = cs.d;
-                } else {
-                    o = xyz(fp.ro[cur][q]);
-                    d = xyz(fp.rd[cur][q]);
-                }
-                const Surface sf = surface_at(fp, h, o, d, fp.nee != 0);
-                diffuse = static_cast<int>(fp.sc.mats[3 * sf.mat].w) == 0;
-                pnt = sf.pnt;
-                nrm = sf.nrm;
-            }
-        }
-        const unsigned long long rows = __ballot(diffuse);
-        if (rows == 0) continue;   // the whole wave agrees
-        if (fp.visits && lane == 0) atomicAdd(fp.visits + 7, static_cast<unsigned long long>(__popcll(rows)));
-        guide::RegActs X;
-        guide::encode_regs(fp.gv, pnt, nrm, X);
-        guide::f16v O[2];
-        guide::forward_regs(guide::LdsWeights{net}, X, O);
-        // The lane holds whole lobes (2 g4 + kh, g4 < 4) of queries c and
-        // 32 + c: each is decoded once here for the shade stage, the softmax
-        // taking the other half's four lobes from the partner lane.
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            float mx = O[nb][3];
-#pragma unroll
-            for (int g4 = 1; g4 < 4; ++g4) mx = fmaxf(mx, O[nb][4 * g4 + 3]);
-            mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-            float e[4], sum = 0;
-#pragma unroll
-            for (int g4 = 0; g4 < 4; ++g4) {
-                e[g4] = __expf(O[nb][4 * g4 + 3] - mx);
-                sum += e[g4];
-            }
-            sum += __shfl_xor(sum, 32, 64);
-            const float inv = __fdividef(1.0f, sum);
-            const int i = 32 * nb + c;
-            if ((rows >> i) & 1ull) {
-                _Float16* out = fp.gout + static_cast<size_t>(base + i) * guide::OUT + 4 * kh;
-#pragma unroll
-                for (int g4 = 0; g4 < 4; ++g4) {
-                    float x, y, sk;
-                    guide::decode_lobe(O[nb][4 * g4], O[nb][4 * g4 + 1], O[nb][4 * g4 + 2], x, y, sk);
-                    const guide::h4 hv{static_cast<_Float16>(x), static_cast<_Float16>(y), static_cast<_Float16>(sk),
-                                       static_cast<_Float16>(e[g4] * inv)};
-                    *reinterpret_cast<guide::h4*>(out + 8 * g4) = hv;
+    uint32_t staged = 0;   // entries in the wave's list (fewer than 64 between runs)
+    // One loop with one call of the network, so it is compiled once: a run
+    // of slots is staged, then a full batch runs, or, once the slots are
+    // done, the last partial one.
+    for (uint32_t base = blockIdx.x * INFER_BLOCK + (threadIdx.x & ~63u);; base += stride) {
+        const bool more = base < n;
+        if (more) {
+            const uint32_t q = base + lane;
+            bool diffuse = false;
+            f3 pnt = mk(0, 0, 0), nrm = mk(0, 0, 1);
+            if (q < n) {
+                const Hit h = fp.hits[q];
+                if (h.kind != 0) {
+                    f3 o, d;
+                    if (PRIMARY) {
+                        const CameraSample cs = camera_sample(fp, fc, q);
+                        o = cs.o;
+                        d = cs.d;
+                    } else {
+                        o = xyz(fp.ro[cur][q]);
+                        d = xyz(fp.rd[cur][q]);
+                    }
+                    const Surface sf = surface_at<true>(fp, h, o, d, fp.nee != 0);
+                    diffuse = static_cast<int>(fp.sc.mats[3 * sf.mat].w) == 0;
+                    pnt = sf.pnt;
+                    nrm = sf.nrm;
                 }
             }
+            const unsigned long long rows = __ballot(diffuse);
+            if (rows == 0) continue;   // the whole wave agrees
+            if (fp.visits && lane == 0) atomicAdd(fp.visits + 7, static_cast<unsigned long long>(__popcll(rows)));
+            if (diffuse) {
+                InferEntry& en = st[staged + static_cast<uint32_t>(__popcll(rows & below))];
+                guide::encode_scalars(fp.gv, pnt, nrm, en.e);
+                en.slot = q;
+            }
+            staged += static_cast<uint32_t>(__popcll(rows));
+            __builtin_amdgcn_wave_barrier();
+            if (staged < 64) continue;
+        } else if (staged == 0) {
+            break;
         }
+        const uint32_t count = staged < 64 ? staged : 64;
+        infer_staged(fp, net, st, count);
+        // The entries past the batch move to the front of the list.
+        const uint32_t rest = staged - count;
+        uint4 m0 = make_uint4(0, 0, 0, 0), m1 = m0;
+        if (lane < rest) {
+            const uint4* from = reinterpret_cast<const uint4*>(st + 64 + lane);
+            m0 = from[0];
+            m1 = from[1];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < rest) {
+            uint4* to = reinterpret_cast<uint4*>(st + lane);
+            to[0] = m0;
+            to[1] = m1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        staged = rest;
+        if (!more) break;
     }
 }
 
