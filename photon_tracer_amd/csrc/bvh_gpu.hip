@@ -79,6 +79,13 @@ struct SmallSeg {   // a segment one wave finishes
 
 constexpr uint32_t NO_DEPTH = 0xffffffffu;   // depth_of of an unused node slot
 
+// The level passes' control words on the device, so the host need not read
+// each level's segments back: the open segments and chunks of the level
+// with parity p, the small segments so far, an error flag (a list would
+// overflow: cannot happen, the lists are sized for disjoint segments), and
+// the node slots the whole tree takes.
+enum : uint32_t { LC_NSEG = 0, LC_NCH = 2, LC_NSMALL = 4, LC_ERR = 5, LC_TOTAL = 6, LC_WORDS = 8 };
+
 struct SegAcc {
     uint32_t key[12];   // box min, box max, centroid min, centroid max
     uint32_t bins[kBinWords];
@@ -266,7 +273,8 @@ __global__ __launch_bounds__(kBlock) void k_setup(const float* __restrict__ boxe
 // ---------------------------------------------------------------------------
 // Level passes over segments longer than kSmall.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_seg_init(SegAcc* acc) {
+__global__ __launch_bounds__(kBlock) void k_seg_init(SegAcc* acc, const uint32_t* lc, int p) {
+    if (blockIdx.x >= lc[LC_NSEG + p]) return;
     SegAcc& a = acc[blockIdx.x];
     for (int i = threadIdx.x; i < 12; i += kBlock) a.key[i] = (i % 6) < 3 ? KEY_POS_INF : KEY_NEG_INF;
     for (int i = threadIdx.x; i < kBinWords; i += kBlock) a.bins[i] = bin_init_word(i);
@@ -279,7 +287,9 @@ __device__ __forceinline__ uint32_t chunk_end(const uint2& ch, const Seg& s) {
 // bvh.hpp:128-132: the segment's box and centroid bounds.
 __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ boxes, const float4* __restrict__ centers,
                                                     const uint32_t* __restrict__ order, const uint2* __restrict__ chunks,
-                                                    const Seg* __restrict__ segs, SegAcc* acc) {
+                                                    const Seg* __restrict__ segs, SegAcc* acc,
+        const uint32_t* lc, int p) {
+    if (blockIdx.x >= lc[LC_NCH + p]) return;
     __shared__ float part[kBlock / 64][12];
     const uint2 ch = chunks[blockIdx.x];
     const Seg s = segs[ch.x];
@@ -318,7 +328,9 @@ __global__ __launch_bounds__(kBlock) void k_bounds(const float* __restrict__ box
 // boxes and counts.
 __global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ boxes, const float4* __restrict__ centers,
                                                  const uint32_t* __restrict__ order, const uint2* __restrict__ chunks,
-                                                 const Seg* __restrict__ segs, SegAcc* acc) {
+                                                 const Seg* __restrict__ segs, SegAcc* acc,
+        const uint32_t* lc, int p) {
+    if (blockIdx.x >= lc[LC_NCH + p]) return;
     __shared__ uint32_t bins[kBinWords];
     const uint2 ch = chunks[blockIdx.x];
     const Seg s = segs[ch.x];
@@ -363,10 +375,10 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ boxes,
 
 // bvh.hpp:126-191 per segment: the node's box, the leaf test, the SAH split
 // and the child pair.
-__global__ __launch_bounds__(64) void k_decide(Seg* segs, const SegAcc* __restrict__ acc, uint32_t nseg, BNode* nodes,
-                                               uint32_t* depth_of, uint32_t* counters) {
+__global__ __launch_bounds__(64) void k_decide(Seg* segs, const SegAcc* __restrict__ acc, const uint32_t* lc, int p,
+                                               BNode* nodes, uint32_t* depth_of, uint32_t* counters) {
     const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= nseg) return;
+    if (i >= lc[LC_NSEG + p]) return;
     Seg s = segs[i];
     const SegAcc& A = acc[i];
     FBox box, cen;
@@ -449,7 +461,9 @@ __device__ __forceinline__ void block_sum2(uint32_t& a, uint32_t& b, uint32_t (*
 __global__ __launch_bounds__(kBlock) void k_part_count(const float4* __restrict__ centers,
                                                         const uint32_t* __restrict__ order,
                                                         const uint2* __restrict__ chunks, const Seg* __restrict__ segs,
-                                                        uint32_t* cnt) {
+                                                        uint32_t* cnt,
+        const uint32_t* lc, int p) {
+    if (blockIdx.x >= lc[LC_NCH + p]) return;
     __shared__ uint32_t part[kBlock / 64][2];
     const uint2 ch = chunks[blockIdx.x];
     const Seg s = segs[ch.x];
@@ -473,7 +487,9 @@ __global__ __launch_bounds__(kBlock) void k_part_count(const float4* __restrict_
 __global__ __launch_bounds__(kBlock) void k_part_pos(const float4* __restrict__ centers,
                                                       const uint32_t* __restrict__ order,
                                                       const uint2* __restrict__ chunks, const Seg* __restrict__ segs,
-                                                      const uint32_t* __restrict__ cnt, uint32_t* pos) {
+                                                      const uint32_t* __restrict__ cnt, uint32_t* pos,
+        const uint32_t* lc, int p) {
+    if (blockIdx.x >= lc[LC_NCH + p]) return;
     __shared__ uint32_t part[kBlock / 64][2];
     const uint2 ch = chunks[blockIdx.x];
     const Seg s = segs[ch.x];
@@ -522,7 +538,9 @@ __global__ __launch_bounds__(kBlock) void k_part_pos(const float4* __restrict__ 
 // right.
 __global__ __launch_bounds__(kBlock) void k_part_swap(const uint2* __restrict__ chunks, const Seg* __restrict__ segs,
                                                        const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ pos,
-                                                       uint32_t* order) {
+                                                       uint32_t* order,
+        const uint32_t* lc, int p) {
+    if (blockIdx.x >= lc[LC_NCH + p]) return;
     __shared__ uint32_t part[kBlock / 64][2];
     const uint2 ch = chunks[blockIdx.x];
     const Seg s = segs[ch.x];
@@ -538,6 +556,143 @@ __global__ __launch_bounds__(kBlock) void k_part_swap(const uint2* __restrict__ 
         order[a] = order[b];
         order[b] = x;
     }
+}
+
+// ---------------------------------------------------------------------------
+// The next level's lists, on the device: the children of the level's split
+// segments in segment order, those longer than kSmall as the next level's
+// segments (with their chunks) and the others appended to the small list,
+// the order the host loop that came before this kernel used. One block of
+// kPlanBlock threads walks the level's segments in tiles.
+// ---------------------------------------------------------------------------
+constexpr int kPlanBlock = 1024;
+
+// Exclusive block-wide prefix sums of three counters, and their totals.
+__device__ __forceinline__ void block_scan3(uint32_t (&v)[3], uint32_t (&tot)[3], uint32_t (*part)[3]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint32_t x = v[k];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        incl[k] = x;
+    }
+    __syncthreads();
+    if (lane == 63)
+        for (int k = 0; k < 3; ++k) part[wave][k] = incl[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        uint32_t before = 0, all = 0;
+        for (int w = 0; w < kPlanBlock / 64; ++w) {
+            const uint32_t t = part[w][k];
+            before += w < wave ? t : 0u;
+            all += t;
+        }
+        v[k] = before + incl[k] - v[k];
+        tot[k] = all;
+    }
+}
+
+__global__ __launch_bounds__(kPlanBlock) void k_plan(const Seg* __restrict__ cur, uint32_t* lc, int p, Seg* next,
+                                                      uint2* next_chunks, SmallSeg* small, uint32_t max_segs,
+                                                      uint32_t max_chunks, uint32_t max_small) {
+    __shared__ uint32_t part[kPlanBlock / 64][3];
+    const uint32_t nseg = lc[LC_NSEG + p];
+    uint32_t n_big = 0, n_ch = 0, n_small = lc[LC_NSMALL];
+    bool overflow = false;
+    for (uint32_t tile = 0; tile < nseg; tile += kPlanBlock) {
+        const uint32_t i = tile + threadIdx.x;
+        uint32_t first[2] = {0, 0}, count[2] = {0, 0}, node[2] = {0, 0}, depth = 0;
+        int kids = 0;
+        if (i < nseg) {
+            const Seg sg = cur[i];
+            if (sg.kind != 0) {
+                kids = 2;
+                depth = sg.depth + 1;
+                first[0] = sg.first;
+                count[0] = sg.mid - sg.first;
+                node[0] = sg.left;
+                first[1] = sg.mid;
+                count[1] = sg.first + sg.count - sg.mid;
+                node[1] = sg.left + 1;
+            }
+        }
+        uint32_t v[3] = {0, 0, 0}, tot[3];   // big children, their chunks, small children
+        for (int c = 0; c < kids; ++c) {
+            if (count[c] > kSmall) {
+                ++v[0];
+                v[1] += (count[c] + kChunk - 1) / kChunk;
+            } else {
+                ++v[2];
+            }
+        }
+        block_scan3(v, tot, part);
+        for (int c = 0; c < kids; ++c) {
+            if (count[c] > kSmall) {
+                const uint32_t at = n_big + v[0]++, ch0 = n_ch + v[1];
+                const uint32_t nc = (count[c] + kChunk - 1) / kChunk;
+                v[1] += nc;
+                if (at >= max_segs || ch0 + nc > max_chunks) {
+                    overflow = true;
+                    continue;
+                }
+                Seg ns{};
+                ns.first = first[c];
+                ns.count = count[c];
+                ns.depth = depth;
+                ns.node = node[c];
+                ns.chunk_begin = ch0;
+                ns.chunk_count = nc;
+                next[at] = ns;
+                for (uint32_t k = 0; k < nc; ++k) next_chunks[ch0 + k] = make_uint2(at, first[c] + k * kChunk);
+            } else {
+                const uint32_t at = n_small + v[2]++;
+                if (at >= max_small) {
+                    overflow = true;
+                    continue;
+                }
+                SmallSeg ss{};
+                ss.first = first[c];
+                ss.count = count[c];
+                ss.depth = depth;
+                ss.node = node[c];
+                small[at] = ss;
+            }
+        }
+        n_big += tot[0];
+        n_ch += tot[1];
+        n_small += tot[2];
+    }
+    if (overflow) lc[LC_ERR] = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        lc[LC_NSEG + (p ^ 1)] = n_big;
+        lc[LC_NCH + (p ^ 1)] = n_ch;
+        lc[LC_NSMALL] = n_small;
+    }
+}
+
+// Each small segment's node range, [base, base + 2 count - 2), after the
+// level passes' nodes (counters[0] of them), in list order; lc[LC_TOTAL]:
+// every node slot the tree takes.
+__global__ __launch_bounds__(kPlanBlock) void k_small_bases(SmallSeg* small, uint32_t* lc, const uint32_t* counters) {
+    __shared__ uint32_t part[kPlanBlock / 64][3];
+    const uint32_t ns = lc[LC_NSMALL];
+    uint32_t carry = counters[0];
+    for (uint32_t tile = 0; tile < ns; tile += kPlanBlock) {
+        const uint32_t i = tile + threadIdx.x;
+        const uint32_t c = i < ns ? small[i].count : 1u;
+        uint32_t v[3] = {2 * c - 2, 0, 0}, tot[3];
+        block_scan3(v, tot, part);
+        if (i < ns) small[i].base = carry + v[0];
+        carry += tot[0];
+    }
+    if (threadIdx.x == 0) lc[LC_TOTAL] = carry;
 }
 
 // ---------------------------------------------------------------------------
@@ -923,9 +1078,9 @@ size_t scratch_bytes(uint32_t n) {
     const size_t max_chunks = n / kChunk + max_segs + 2;
     return DevMem::round(sizeof(float4) * n) + DevMem::round(4 * size_t(n)) + DevMem::round(sizeof(BNode) * cap) +
            3 * DevMem::round(4 * cap) + DevMem::round(sizeof(int4) * cap) + DevMem::round(16) +
-           DevMem::round(sizeof(Seg) * max_segs) + DevMem::round(sizeof(SegAcc) * max_segs) +
-           DevMem::round(sizeof(uint2) * max_chunks) + DevMem::round(8 * max_chunks) +
-           DevMem::round(sizeof(SmallSeg) * size_t(n));
+           2 * DevMem::round(sizeof(Seg) * max_segs) + DevMem::round(sizeof(SegAcc) * max_segs) +
+           2 * DevMem::round(sizeof(uint2) * max_chunks) + DevMem::round(8 * max_chunks) +
+           DevMem::round(sizeof(SmallSeg) * size_t(n)) + DevMem::round(4 * LC_WORDS);
 }
 
 hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_nodes, uint32_t* n_nodes,
@@ -939,7 +1094,7 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
     mem.base = static_cast<char*>(d_scratch);
     mem.size = scratch_size;
     float4* centers;
-    uint32_t *pos, *depth_of, *counters, *wsize, *cnt;
+    uint32_t *pos, *depth_of, *counters, *wsize, *cnt, *lc;
     BNode* nodes;
     int4* clist;
     int32_t* index;
@@ -951,19 +1106,48 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
     BVH_TRY(mem.get(&wsize, cap));
     BVH_TRY(mem.get(&index, cap));
     BVH_TRY(mem.get(&counters, 4));
-    Seg* segs;
+    Seg* segs[2];
     SegAcc* acc;
-    uint2* chunks;
-    BVH_TRY(mem.get(&segs, max_segs));
+    uint2* chunks[2];
+    BVH_TRY(mem.get(&segs[0], max_segs));
+    BVH_TRY(mem.get(&segs[1], max_segs));
     BVH_TRY(mem.get(&acc, max_segs));
-    BVH_TRY(mem.get(&chunks, max_chunks));
+    BVH_TRY(mem.get(&chunks[0], max_chunks));
+    BVH_TRY(mem.get(&chunks[1], max_chunks));
     BVH_TRY(mem.get(&cnt, 2 * max_chunks));
     SmallSeg* d_small;   // small segments are disjoint and non-empty: at most n
     BVH_TRY(mem.get(&d_small, n));
+    BVH_TRY(mem.get(&lc, LC_WORDS));
 
     BVH_TRY(hipMemsetAsync(counters, 0, 4 * sizeof(uint32_t), st));
     k_setup<<<blocks(n), kBlock, 0, st>>>(d_boxes, n, centers, d_order, counters, depth_of, cap);
     BVH_TRY(hipGetLastError());
+
+    // The root segment: a level pass's if longer than kSmall, else the one
+    // small segment. The level passes then run without the host: each level
+    // ends with k_plan writing the next level's lists and counts, and the
+    // kernels of a level read their counts from lc (grids sized for the
+    // most a level can hold: at most 2^L segments at level L, and a chunk
+    // per kChunk primitives plus one per segment).
+    uint32_t h_lc[LC_WORDS] = {};
+    Seg root{};
+    root.count = n;
+    std::vector<uint2> root_chunks;
+    if (n > kSmall) {
+        root.chunk_count = (n + kChunk - 1) / kChunk;
+        for (uint32_t k = 0; k < root.chunk_count; ++k) root_chunks.push_back(make_uint2(0, k * kChunk));
+        h_lc[LC_NSEG] = 1;
+        h_lc[LC_NCH] = root.chunk_count;
+        BVH_TRY(hipMemcpyAsync(segs[0], &root, sizeof(Seg), hipMemcpyHostToDevice, st));
+        BVH_TRY(hipMemcpyAsync(chunks[0], root_chunks.data(), root_chunks.size() * sizeof(uint2),
+                               hipMemcpyHostToDevice, st));
+    } else {
+        SmallSeg one{};
+        one.count = n;
+        h_lc[LC_NSMALL] = 1;
+        BVH_TRY(hipMemcpyAsync(d_small, &one, sizeof(SmallSeg), hipMemcpyHostToDevice, st));
+    }
+    BVH_TRY(hipMemcpyAsync(lc, h_lc, sizeof(h_lc), hipMemcpyHostToDevice, st));
     // Boxes with a NaN or a -0.0 coordinate stop here (the caller decides).
     uint32_t flags = 0;
     BVH_TRY(hipMemcpyAsync(&flags, counters + 2, 4, hipMemcpyDeviceToHost, st));
@@ -972,79 +1156,51 @@ hipError_t build(const float* d_boxes, uint32_t n, uint32_t* d_order, void* d_no
     *n_nodes = 0;
     if (flags) return hipSuccess;
 
-    std::vector<SmallSeg> small;
-    uint32_t level_nodes = 1;   // the root, then two per split of a level pass
-    std::vector<Seg> cur;
-    auto add = [&](std::vector<Seg>& next, uint32_t first, uint32_t count, uint32_t depth, uint32_t node) {
-        if (count > kSmall) {
-            Seg s{};
-            s.first = first;
-            s.count = count;
-            s.depth = depth;
-            s.node = node;
-            next.push_back(s);
-        } else {
-            SmallSeg s{};
-            s.first = first;
-            s.count = count;
-            s.depth = depth;
-            s.node = node;
-            small.push_back(s);
+    if (n > kSmall) {
+        // The host looks at the counts only now and then: first after the
+        // levels a balanced split of n would take, then every two levels
+        // (a level with nothing open costs a few empty launches).
+        uint32_t check_at = 1;
+        while ((uint64_t(kSmall) << check_at) < n) ++check_at;
+        int p = 0;
+        for (uint32_t level = 0;; ++level) {
+            if (level > kMaxDepth) return hipErrorInvalidValue;   // cannot happen: depth is capped at kMaxDepth
+            const uint32_t seg_grid = level < 31 ? std::min<uint32_t>(max_segs, 1u << level) : max_segs;
+            const uint32_t ch_grid = static_cast<uint32_t>(
+                std::min<uint64_t>(max_chunks, uint64_t(n) / kChunk + 1 + seg_grid));
+            k_seg_init<<<seg_grid, kBlock, 0, st>>>(acc, lc, p);
+            k_bounds<<<ch_grid, kBlock, 0, st>>>(d_boxes, centers, d_order, chunks[p], segs[p], acc, lc, p);
+            k_bin<<<ch_grid, kBlock, 0, st>>>(d_boxes, centers, d_order, chunks[p], segs[p], acc, lc, p);
+            k_decide<<<(seg_grid + 63) / 64, 64, 0, st>>>(segs[p], acc, lc, p, nodes, depth_of, counters);
+            k_part_count<<<ch_grid, kBlock, 0, st>>>(centers, d_order, chunks[p], segs[p], cnt, lc, p);
+            k_part_pos<<<ch_grid, kBlock, 0, st>>>(centers, d_order, chunks[p], segs[p], cnt, pos, lc, p);
+            k_part_swap<<<ch_grid, kBlock, 0, st>>>(chunks[p], segs[p], cnt, pos, d_order, lc, p);
+            k_plan<<<1, kPlanBlock, 0, st>>>(segs[p], lc, p, segs[p ^ 1], chunks[p ^ 1], d_small, max_segs, max_chunks,
+                                             n);
+            BVH_TRY(hipGetLastError());
+            p ^= 1;
+            if (level + 1 >= check_at) {
+                BVH_TRY(hipMemcpyAsync(h_lc, lc, sizeof(h_lc), hipMemcpyDeviceToHost, st));
+                BVH_TRY(hipStreamSynchronize(st));
+                if (h_lc[LC_ERR]) return hipErrorInvalidValue;
+                if (h_lc[LC_NSEG + p] == 0) break;
+                check_at = level + 3;
+            }
         }
-    };
-    add(cur, 0, n, 0, 0);
-    std::vector<uint2> chunk_list;
-    while (!cur.empty()) {
-        const uint32_t nseg = static_cast<uint32_t>(cur.size());
-        if (nseg > max_segs) return hipErrorInvalidValue;   // cannot happen: segments are disjoint and > kSmall
-        chunk_list.clear();
-        for (uint32_t i = 0; i < nseg; ++i) {
-            cur[i].chunk_begin = static_cast<uint32_t>(chunk_list.size());
-            for (uint32_t p = cur[i].first; p < cur[i].first + cur[i].count; p += kChunk) chunk_list.push_back(make_uint2(i, p));
-            cur[i].chunk_count = static_cast<uint32_t>(chunk_list.size()) - cur[i].chunk_begin;
-        }
-        const uint32_t nch = static_cast<uint32_t>(chunk_list.size());
-        if (nch > max_chunks) return hipErrorInvalidValue;
-        BVH_TRY(hipMemcpyAsync(segs, cur.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice, st));
-        BVH_TRY(hipMemcpyAsync(chunks, chunk_list.data(), nch * sizeof(uint2), hipMemcpyHostToDevice, st));
-        k_seg_init<<<nseg, kBlock, 0, st>>>(acc);
-        k_bounds<<<nch, kBlock, 0, st>>>(d_boxes, centers, d_order, chunks, segs, acc);
-        k_bin<<<nch, kBlock, 0, st>>>(d_boxes, centers, d_order, chunks, segs, acc);
-        k_decide<<<(nseg + 63) / 64, 64, 0, st>>>(segs, acc, nseg, nodes, depth_of, counters);
-        k_part_count<<<nch, kBlock, 0, st>>>(centers, d_order, chunks, segs, cnt);
-        k_part_pos<<<nch, kBlock, 0, st>>>(centers, d_order, chunks, segs, cnt, pos);
-        k_part_swap<<<nch, kBlock, 0, st>>>(chunks, segs, cnt, pos, d_order);
-        BVH_TRY(hipGetLastError());
-        BVH_TRY(hipMemcpyAsync(cur.data(), segs, nseg * sizeof(Seg), hipMemcpyDeviceToHost, st));
-        BVH_TRY(hipStreamSynchronize(st));
-        std::vector<Seg> next;
-        for (const Seg& s : cur) {
-            if (s.kind == 0) continue;
-            level_nodes += 2;
-            add(next, s.first, s.mid - s.first, s.depth + 1, s.left);
-            add(next, s.mid, s.first + s.count - s.mid, s.depth + 1, s.left + 1);
-        }
-        cur.swap(next);
     }
 
-    uint32_t total = level_nodes;
-    for (SmallSeg& s : small) {
-        s.base = total;
-        total += 2 * s.count - 2;
-    }
-    if (total > cap) return hipErrorInvalidValue;
-    if (!small.empty()) {
-        if (small.size() > n) return hipErrorInvalidValue;
-        BVH_TRY(hipMemcpyAsync(d_small, small.data(), small.size() * sizeof(SmallSeg), hipMemcpyHostToDevice, st));
-        k_small<<<static_cast<uint32_t>(small.size()), 64, 0, st>>>(d_boxes, centers, d_order, d_small, nodes, depth_of,
-                                                                    counters);
-        BVH_TRY(hipGetLastError());
-    }
+    k_small_bases<<<1, kPlanBlock, 0, st>>>(d_small, lc, counters);
+    BVH_TRY(hipMemcpyAsync(h_lc, lc, sizeof(h_lc), hipMemcpyDeviceToHost, st));
+    BVH_TRY(hipStreamSynchronize(st));
+    const uint32_t n_small = h_lc[LC_NSMALL], total = h_lc[LC_TOTAL];
+    if (h_lc[LC_ERR] || n_small == 0 || n_small > n || total > cap) return hipErrorInvalidValue;
+    k_small<<<n_small, 64, 0, st>>>(d_boxes, centers, d_order, d_small, nodes, depth_of, counters);
+    BVH_TRY(hipGetLastError());
     uint32_t hc[2];
     BVH_TRY(hipMemcpyAsync(hc, counters, 8, hipMemcpyDeviceToHost, st));
     BVH_TRY(hipStreamSynchronize(st));
     const uint32_t deepest = hc[1];
-    if (hc[0] != level_nodes || deepest >= kMaxDepth) return hipErrorInvalidValue;
+    if (deepest >= kMaxDepth) return hipErrorInvalidValue;
 
     k_clist<<<blocks(total), kBlock, 0, st>>>(nodes, depth_of, total, clist);
     k_index_init<<<blocks(total), kBlock, 0, st>>>(total, index);
