@@ -405,7 +405,12 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, const Scene
             const bool nodes_left = __ballot(live && !at_leaf) != 0;
             if (at_leaf && nodes_left && n_leaf < leaf_batch<PRIMARY>::lanes) step = false;
         }
-        if (step && walk_step<COUNT, INST>(sc, w, any, st, &vis)) {
+        // Waves stepping only light samples (any hit) push hit children
+        // without ranking them. Scenes with instances keep the ranking: the
+        // extra ballot made the instancing scene 1.4% slower, against 0.7%
+        // faster for the 1M-triangle mesh.
+        const bool ordered = PRIMARY || INST || __ballot(step && !any) != 0;
+        if (step && walk_step<COUNT, INST>(sc, w, any, st, &vis, ordered)) {
             live = false;
             done = true;
         }

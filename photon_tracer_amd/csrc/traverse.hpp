@@ -105,9 +105,15 @@ typedef float lds_f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float4 as_f4(lds_f4 v) { return make_float4(v.x, v.y, v.z, v.w); }
 
+//
+// ordered = false is for any-hit queries: whether anything is hit in
+// [tmin, tmax] does not depend on the order the children are taken in (no
+// hit moves tmax before the query ends), so the hit children are pushed in
+// child order and the last is taken next, without ranking their entry
+// distances. The caller passes the same value on every lane of a wave.
 template <class Stack>
 __device__ __forceinline__ bool visit_node(const SceneView& sc, uint32_t idx, f3 o, f3 inv, float tmin, float tmax,
-                                           Stack& st, uint2& next) {
+                                           Stack& st, uint2& next, bool ordered = true) {
     float4 lx, ly, lz, hx, hy, hz;
     uint4 cd;   // first | count << 28 (the device copy of the nodes)
     if (idx < sc.n_cached) {
@@ -159,14 +165,20 @@ __device__ __forceinline__ bool visit_node(const SceneView& sc, uint32_t idx, f3
     // Position of each hit child in the farthest-first order: those entered
     // later come first, and on equal entry the lower child number first.
     int rank[4] = {0, 0, 0, 0};
+    if (ordered) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < 4; ++a) {
 #pragma unroll
-        for (int b2 = a + 1; b2 < 4; ++b2) {
-            const bool later = tn[b2] > tn[a];     // b2 is pushed before a
-            rank[a] += (h[b2] && later) ? 1 : 0;
-            rank[b2] += (h[a] && !later) ? 1 : 0;
+            for (int b2 = a + 1; b2 < 4; ++b2) {
+                const bool later = tn[b2] > tn[a];     // b2 is pushed before a
+                rank[a] += (h[b2] && later) ? 1 : 0;
+                rank[b2] += (h[a] && !later) ? 1 : 0;
+            }
         }
+    } else {
+        rank[1] = h[0] ? 1 : 0;
+        rank[2] = rank[1] + (h[1] ? 1 : 0);
+        rank[3] = rank[2] + (h[2] ? 1 : 0);
     }
     const int n = (h[0] ? 1 : 0) + (h[1] ? 1 : 0) + (h[2] ? 1 : 0) + (h[3] ? 1 : 0);
     if (n == 0) return false;
@@ -225,7 +237,7 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                have = visit_node(sc, first, o, inv, tmin, tmax, st, e);
+                have = visit_node(sc, first, o, inv, tmin, tmax, st, e, !ANY);
                 continue;
             }
             for (uint32_t i = first; i < first + count; ++i) {
@@ -298,8 +310,8 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                if (in_blas) have = visit_node(sc, first, lo, linv, tmin, tmax, st, e);
-                else have = visit_node(sc, first, o, inv, tmin, tmax, st, e);
+                if (in_blas) have = visit_node(sc, first, lo, linv, tmin, tmax, st, e, !ANY);
+                else have = visit_node(sc, first, o, inv, tmin, tmax, st, e, !ANY);
                 continue;
             }
             if (COUNT) vis->leafrefs += count;
@@ -457,7 +469,8 @@ __device__ __forceinline__ bool walk_advance(const SceneView& sc, Walk<INST>& w,
 // when the walk has ended (every tree walked, or an any-hit query found a
 // hit); kind != 0 then says something was hit.
 template <bool COUNT, bool INST = false, class Stack>
-__device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bool any, Stack& st, Visits* vis) {
+__device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bool any, Stack& st, Visits* vis,
+                                          bool ordered = true) {
     w.have = false;
     const uint32_t first = w.e.x & FIRST_MASK, count = w.e.x >> COUNT_SHIFT;
     if (count == 0) {
@@ -465,9 +478,9 @@ __device__ __forceinline__ bool walk_step(const SceneView& sc, Walk<INST>& w, bo
         bool in_blas = false;
         if constexpr (INST) in_blas = w.in_blas;
         if (in_blas) {
-            if constexpr (INST) w.have = visit_node(sc, first, w.lo, w.linv, w.tmin, w.tmax, st, w.e);
+            if constexpr (INST) w.have = visit_node(sc, first, w.lo, w.linv, w.tmin, w.tmax, st, w.e, ordered);
         } else {
-            w.have = visit_node(sc, first, w.o, w.inv, w.tmin, w.tmax, st, w.e);
+            w.have = visit_node(sc, first, w.o, w.inv, w.tmin, w.tmax, st, w.e, ordered);
         }
         return walk_advance<COUNT, INST>(sc, w, st, vis);
     }
