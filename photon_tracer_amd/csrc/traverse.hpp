@@ -200,7 +200,9 @@ __device__ __forceinline__ bool visit_node(const SceneView& sc, uint32_t idx, f3
 __device__ __forceinline__ f3 safe_rcp3(f3 d) { return {rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z)}; }
 
 // Returns whether something was hit. With ANY, stops at the first hit
-// (geometry::occluded); otherwise fills `hit` with the nearest.
+// (geometry::occluded); otherwise fills `hit` with the nearest. Any-hit
+// walks of scenes without instances push children unranked; with instances
+// the nearest-first order finds the occluder sooner and stays.
 // INST = false is the variant for scenes without instances: the instance
 // walk, and the registers it keeps live, are compiled out.
 template <bool ANY, bool COUNT = false, bool INST = true, class Stack>
@@ -237,7 +239,7 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                have = visit_node(sc, first, o, inv, tmin, tmax, st, e, !ANY);
+                have = visit_node(sc, first, o, inv, tmin, tmax, st, e, !ANY || INST);
                 continue;
             }
             for (uint32_t i = first; i < first + count; ++i) {
@@ -310,8 +312,8 @@ __device__ __forceinline__ bool trace_ray(const SceneView& sc, f3 o, f3 d, float
             const uint32_t first = e.x & FIRST_MASK, count = e.x >> COUNT_SHIFT;
             if (count == 0) {
                 if (COUNT) ++vis->nodes;
-                if (in_blas) have = visit_node(sc, first, lo, linv, tmin, tmax, st, e, !ANY);
-                else have = visit_node(sc, first, o, inv, tmin, tmax, st, e, !ANY);
+                if (in_blas) have = visit_node(sc, first, lo, linv, tmin, tmax, st, e, true);
+                else have = visit_node(sc, first, o, inv, tmin, tmax, st, e, true);
                 continue;
             }
             if (COUNT) vis->leafrefs += count;
