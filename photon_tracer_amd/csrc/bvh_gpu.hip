@@ -24,6 +24,7 @@
 #include "bvh_gpu.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace ptd {
@@ -183,26 +184,58 @@ __device__ __forceinline__ void sweep_axis(const uint32_t* bins, float parent_ar
     }
 }
 
-__device__ __forceinline__ float wave_min(float x) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const float y = __shfl_xor(x, m, 64);
-        x = y < x ? y : x;
-    }
+// Lane exchanges within rows of 16 lanes by DPP (a VALU operand modifier,
+// no trip through LDS): quad_perm swaps of neighbours one and two apart,
+// mirrors within half rows and rows, and shifts along a row. A source lane
+// outside the row leaves the lane its own value. Across rows, shuffles.
+constexpr int DPP_QUAD_XOR1 = 0xB1;         // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;         // quad_perm [2, 3, 0, 1]
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;  // lane i <- 7 - i within each half row
+constexpr int DPP_ROW_MIRROR = 0x140;       // lane i <- 15 - i within each row
+constexpr int DPP_ROW_SHL = 0x100;          // + n: lane i <- lane i + n of its row
+constexpr int DPP_ROW_SHR = 0x110;          // + n: lane i <- lane i - n of its row
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t x) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(x), static_cast<int>(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __uint_as_float(dpp_u<CTRL>(__float_as_uint(x)));
+}
+
+// Reductions over the wave: every lane gets the result. The lanes are
+// combined in pairs whose values are the same whatever order min, max and
+// integer sums take them in.
+template <class Op, class T, class X>
+__device__ __forceinline__ T wave_reduce(T x, Op op, X exch) {
+    x = op(x, exch(std::integral_constant<int, DPP_QUAD_XOR1>(), x));
+    x = op(x, exch(std::integral_constant<int, DPP_QUAD_XOR2>(), x));
+    x = op(x, exch(std::integral_constant<int, DPP_ROW_HALF_MIRROR>(), x));
+    x = op(x, exch(std::integral_constant<int, DPP_ROW_MIRROR>(), x));
+    x = op(x, __shfl_xor(x, 16, 64));
+    x = op(x, __shfl_xor(x, 32, 64));
     return x;
+}
+struct FExch {
+    template <int C>
+    __device__ __forceinline__ float operator()(std::integral_constant<int, C>, float v) const { return dpp_f<C>(v); }
+};
+struct UExch {
+    template <int C>
+    __device__ __forceinline__ uint32_t operator()(std::integral_constant<int, C>, uint32_t v) const {
+        return dpp_u<C>(v);
+    }
+};
+__device__ __forceinline__ float wave_min(float x) {
+    return wave_reduce(x, [](float a, float b) { return b < a ? b : a; }, FExch());
 }
 __device__ __forceinline__ float wave_max(float x) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const float y = __shfl_xor(x, m, 64);
-        x = x < y ? y : x;
-    }
-    return x;
+    return wave_reduce(x, [](float a, float b) { return a < b ? b : a; }, FExch());
 }
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, 64);
-    return x;
+    return wave_reduce(x, [](uint32_t a, uint32_t b) { return a + b; }, UExch());
 }
 
 // Box and centroid bounds of 12 running values: [0,3) box min, [3,6) box
@@ -816,16 +849,19 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
         }
         suf = pre;
         uint32_t sc = pc;
-#pragma unroll
-        for (int dl = 1; dl < kBins; dl <<= 1) {
+        // One step of the scans: lanes dl or more into the row take the
+        // prefix dl lanes before them, lanes dl or more from its end the
+        // suffix dl lanes after them (row shifts; a row is an axis).
+        auto scan_step = [&](auto dlc) {
+            constexpr int dl = decltype(dlc)::value;
             FBox up, dn;
             for (int q = 0; q < 3; ++q) {
-                up.lo[q] = __shfl_up(pre.lo[q], dl, kBins);
-                up.hi[q] = __shfl_up(pre.hi[q], dl, kBins);
-                dn.lo[q] = __shfl_down(suf.lo[q], dl, kBins);
-                dn.hi[q] = __shfl_down(suf.hi[q], dl, kBins);
+                up.lo[q] = dpp_f<DPP_ROW_SHR + dl>(pre.lo[q]);
+                up.hi[q] = dpp_f<DPP_ROW_SHR + dl>(pre.hi[q]);
+                dn.lo[q] = dpp_f<DPP_ROW_SHL + dl>(suf.lo[q]);
+                dn.hi[q] = dpp_f<DPP_ROW_SHL + dl>(suf.hi[q]);
             }
-            const uint32_t upc = __shfl_up(pc, dl, kBins), dnc = __shfl_down(sc, dl, kBins);
+            const uint32_t upc = dpp_u<DPP_ROW_SHR + dl>(pc), dnc = dpp_u<DPP_ROW_SHL + dl>(sc);
             if (sb >= dl) {
                 box_grow(pre, up.lo, up.hi);
                 pc += upc;
@@ -834,13 +870,18 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
                 box_grow(suf, dn.lo, dn.hi);
                 sc += dnc;
             }
-        }
+        };
+        static_assert(kBins == 16, "one bin a lane of a 16-lane row");
+        scan_step(std::integral_constant<int, 1>());
+        scan_step(std::integral_constant<int, 2>());
+        scan_step(std::integral_constant<int, 4>());
+        scan_step(std::integral_constant<int, 8>());
         FBox right;
         for (int q = 0; q < 3; ++q) {
-            right.lo[q] = __shfl_down(suf.lo[q], 1, kBins);
-            right.hi[q] = __shfl_down(suf.hi[q], 1, kBins);
+            right.lo[q] = dpp_f<DPP_ROW_SHL + 1>(suf.lo[q]);
+            right.hi[q] = dpp_f<DPP_ROW_SHL + 1>(suf.hi[q]);
         }
-        const uint32_t rc = __shfl_down(sc, 1, kBins);
+        const uint32_t rc = dpp_u<DPP_ROW_SHL + 1>(sc);
         float cand_cost = __builtin_inff();
         int cand = 64;
         if (axis_ok && sb < kBins - 1 && pc != 0 && rc != 0) {
@@ -851,18 +892,28 @@ __global__ __launch_bounds__(64) void k_small(const float* __restrict__ boxes, c
                 cand = lane;
             }
         }
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-            const float oc = __shfl_xor(cand_cost, m, 64);
-            const int oi = __shfl_xor(cand, m, 64);
+        // The least (cost, lane) over the wave: the pairs are combined in an
+        // order that does not change the least.
+        auto take = [&](float oc, int oi) {
             if (oi < 64 && (cand >= 64 || oc < cand_cost || (oc == cand_cost && oi < cand))) {
                 cand_cost = oc;
                 cand = oi;
             }
-        }
+        };
+        auto take_dpp = [&](auto ctrl) {
+            constexpr int C = decltype(ctrl)::value;
+            take(dpp_f<C>(cand_cost), static_cast<int>(dpp_u<C>(static_cast<uint32_t>(cand))));
+        };
+        take_dpp(std::integral_constant<int, DPP_QUAD_XOR1>());
+        take_dpp(std::integral_constant<int, DPP_QUAD_XOR2>());
+        take_dpp(std::integral_constant<int, DPP_ROW_HALF_MIRROR>());
+        take_dpp(std::integral_constant<int, DPP_ROW_MIRROR>());
+        take(__shfl_xor(cand_cost, 16, 64), __shfl_xor(cand, 16, 64));
+        take(__shfl_xor(cand_cost, 32, 64), __shfl_xor(cand, 32, 64));
+        cand = __builtin_amdgcn_readfirstlane(cand);   // the same on every lane
         const float best_cost = cand < 64 ? cand_cost : __builtin_inff();
         const int best_axis = cand < 64 ? cand >> 4 : -1, best_bin = cand & 15;
-        const uint32_t left_count = __shfl(pc, cand < 64 ? cand : 0, 64);
+        const uint32_t left_count = __builtin_amdgcn_readlane(pc, cand < 64 ? cand : 0);
         if (c <= kMaxLeaf && !(best_cost < c * kIntersectionCost)) {
             if (lane == 0) {
                 nodes[node].first = g0 + f;
