@@ -4,6 +4,7 @@
 // bvh_gpu.hip.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -45,26 +46,50 @@ Pool& pool() {
     return *p;
 }
 
-// memcpy split over host threads: the destinations here are fresh
-// allocations, whose first touch (a page fault per 4 KB) costs more than
-// the copy itself; spread over threads, the faults are taken in parallel.
-void parallel_copy(void* dst, const void* src, size_t bytes) {
+// fn(lo, hi) over [0, bytes) in pieces of at least 4 MB on up to 16 host
+// threads (one piece: the calling thread alone).
+template <class Fn>
+void parallel_bytes(size_t bytes, Fn fn) {
     constexpr size_t kPiece = size_t(4) << 20;
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t pieces = std::min<size_t>(std::min(hw, 16u), bytes / kPiece);
     if (pieces < 2) {
-        if (bytes) std::memcpy(dst, src, bytes);
+        if (bytes) fn(size_t(0), bytes);
         return;
     }
-    auto part = [&](size_t k) {
-        const size_t lo = bytes * k / pieces, hi = bytes * (k + 1) / pieces;
-        std::memcpy(static_cast<char*>(dst) + lo, static_cast<const char*>(src) + lo, hi - lo);
-    };
+    auto part = [&](size_t k) { fn(bytes * k / pieces, bytes * (k + 1) / pieces); };
     std::vector<std::thread> th;
     th.reserve(pieces - 1);
     for (size_t k = 1; k < pieces; ++k) th.emplace_back(part, k);
     part(0);
     for (std::thread& t : th) t.join();
+}
+
+// memcpy split over host threads: the destinations here are fresh
+// allocations, whose first touch (a page fault per 4 KB) costs more than
+// the copy itself; spread over threads, the faults are taken in parallel.
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    parallel_bytes(bytes, [&](size_t lo, size_t hi) {
+        std::memcpy(static_cast<char*>(dst) + lo, static_cast<const char*>(src) + lo, hi - lo);
+    });
+}
+
+// The BOX_* flags of n boxes in host memory (a NaN coordinate, a -0.0
+// coordinate), scanned branch-free on host threads, so bad boxes are refused
+// before any device call.
+uint32_t host_box_flags(const float* boxes, size_t n) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(boxes);
+    std::atomic<uint32_t> flags{0};
+    parallel_bytes(24 * n, [&](size_t lo, size_t hi) {
+        uint32_t nan = 0, nz = 0;
+        for (size_t i = lo / 4; i < hi / 4; ++i) {
+            const uint32_t u = w[i];
+            nan |= (u & 0x7fffffffu) > 0x7f800000u ? 1u : 0u;
+            nz |= u == 0x80000000u ? 1u : 0u;
+        }
+        flags.fetch_or((nan ? ptd::bvhg::BOX_NAN : 0u) | (nz ? ptd::bvhg::BOX_NEG_ZERO : 0u));
+    });
+    return flags.load();
 }
 
 // The device build over n caller boxes (host memory, 6 floats each): on
@@ -75,11 +100,15 @@ void parallel_copy(void* dst, const void* src, size_t bytes) {
 // comes first in the reference's order, while the device reductions (atomics
 // on order-preserving keys, shuffles) always pick one sign. With a NaN,
 // std::min's result depends on the order it is taken in, so the parallel
-// build could not promise the host's tree: refused. The boxes are checked
-// on the device as the build starts (one pass over them), not on the host.
+// build could not promise the host's tree: refused, before any device call.
+// The build checks the boxes again on the device as it starts.
 bool device_build(Pool& P, const float* boxes, size_t n, uint32_t* k) {
     if (n > ptd::bvhg::kMaxDeviceBoxes)
         throw pt::status_error(PT_ERR_LIMIT, "device BVH build: more than 2^28 primitives");
+    const uint32_t host_flags = host_box_flags(boxes, n);
+    if (host_flags & ptd::bvhg::BOX_NAN)
+        throw pt::status_error(PT_ERR_INVALID, "device BVH build: a box has a NaN coordinate");
+    if (host_flags & ptd::bvhg::BOX_NEG_ZERO) return false;
     P.boxes.ensure(24 * n);
     P.order.ensure(4 * n);
     P.nodes.ensure(sizeof(pt::bvh_node) * n);
