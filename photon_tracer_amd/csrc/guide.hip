@@ -131,15 +131,31 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideView g, const float4* s
     if (threadIdx.x == 0) losses[blockIdx.x] = loss;
 }
 
-// Sums the blocks' gradient rows in block order and scales by inv_count.
-// With grad_out: writes the gradient and stops. Otherwise one Adam step
-// (Kingma and Ba) on the FP32 masters, then the FP16 mirrors.
-__global__ void k_guide_update(GuideParamsDev pd, const float* partials, uint32_t nblocks, float inv_count,
-                               AdamConfig cfg, float* grad_out) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= static_cast<uint32_t>(N_PARAMS)) return;
+// Sums the blocks' gradient rows and scales by inv_count. With grad_out:
+// writes the gradient and stops. Otherwise one Adam step (Kingma and Ba) on
+// the FP32 masters, then the FP16 mirrors. A block takes UPDATE_PARAMS
+// parameters; its UPDATE_GROUPS groups of threads each add a run of rows in
+// row order, and the groups' sums are added in group order: a fixed order,
+// so the step is deterministic, with many blocks and short runs (one thread
+// per parameter over all rows left most of the GPU idle, the update being a
+// chain of dependent loads).
+constexpr int UPDATE_PARAMS = 16, UPDATE_GROUPS = 16;
+__global__ __launch_bounds__(UPDATE_PARAMS * UPDATE_GROUPS) void k_guide_update(GuideParamsDev pd, const float* partials,
+                                                                                uint32_t nblocks, float inv_count,
+                                                                                AdamConfig cfg, float* grad_out) {
+    __shared__ float part[UPDATE_GROUPS][UPDATE_PARAMS];
+    const uint32_t jj = threadIdx.x % UPDATE_PARAMS, grp = threadIdx.x / UPDATE_PARAMS;
+    const uint32_t j = blockIdx.x * UPDATE_PARAMS + jj;
+    const uint32_t run = (nblocks + UPDATE_GROUPS - 1) / UPDATE_GROUPS;
+    const uint32_t b0 = grp * run, b1 = min(nblocks, b0 + run);
     float gsum = 0;
-    for (uint32_t b = 0; b < nblocks; ++b) gsum += partials[static_cast<size_t>(b) * N_PARAMS + j];
+    if (j < static_cast<uint32_t>(N_PARAMS))
+        for (uint32_t b = b0; b < b1; ++b) gsum += partials[static_cast<size_t>(b) * N_PARAMS + j];
+    part[grp][jj] = gsum;
+    __syncthreads();
+    if (grp != 0 || j >= static_cast<uint32_t>(N_PARAMS)) return;
+    gsum = part[0][jj];
+    for (int k = 1; k < UPDATE_GROUPS; ++k) gsum += part[k][jj];
     const float gr = gsum * inv_count;
     if (grad_out) {
         grad_out[j] = gr;
@@ -199,7 +215,8 @@ hipError_t launch_guide_train(const GuideView& g, const float4* samples, uint32_
 
 hipError_t launch_guide_update(const GuideParamsDev& pd, const float* partials, uint32_t nblocks, float inv_count,
                                const AdamConfig& cfg, float* grad_out, hipStream_t stream) {
-    k_guide_update<<<blocks_of(N_PARAMS, 256), 256, 0, stream>>>(pd, partials, nblocks, inv_count, cfg, grad_out);
+    k_guide_update<<<blocks_of(N_PARAMS, UPDATE_PARAMS), UPDATE_PARAMS * UPDATE_GROUPS, 0, stream>>>(
+        pd, partials, nblocks, inv_count, cfg, grad_out);
     return hipGetLastError();
 }
 
