@@ -524,6 +524,7 @@ struct ShadeResult {
     bool cont, shadow, backface;
     f3 pnt, dir, sh_d, sh_l;
     float sh_tmax;
+    uint32_t light;   // the light sample's light (its index in the light list)
 };
 
 // The hit record of a ray's closest hit (geometry.hpp:135-151,
@@ -624,6 +625,7 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
     r.cont = r.shadow = r.backface = false;
     r.pnt = r.dir = r.sh_d = r.sh_l = mk(0, 0, 0);
     r.sh_tmax = 0;
+    r.light = 0;
     const bool was_sampled = ps.sampled;
     ps.sampled = false;
     if (h.kind == 0) {
@@ -726,6 +728,7 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
         // direct_light (renderer.hpp:41-80)
         const uint32_t nl = lv.n_lights;
         const uint32_t pick = min(static_cast<uint32_t>(rand01(ps.key, ps.cnt) * nl), nl - 1);
+        r.light = pick;
         const uint32_t ls = lv.lights[pick];
         const float4 lcr = sc.spheres[2 * ls];
         const uint32_t lmat = __float_as_uint(sc.spheres[2 * ls + 1].x);
@@ -950,14 +953,17 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     FrameConst fc;
     if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1, nxt = cur ^ 1;
-    __shared__ BlockAppend<3> sh;
+    __shared__ BlockAppend<4> sh;
     // GUIDED: each thread's decoded mixture (32 FP16 values), copied from HBM with four
     // 16-byte loads into an LDS row of 17 words (an odd stride: the lobe loops'
     // reads, one output at a time, fall in distinct banks), instead of
     // gathering them two bytes at a time from HBM inside the lobe loops.
     constexpr int GROW = guide::OUT / 2 + 1;
     __shared__ uint32_t gtile[GUIDED ? SHADE_BLOCK * GROW : 1];
-    uint32_t* const counters[3] = {&fp.ctrl[b + 1].n_rays, &c->n_shadow, &c->n_backface};
+    // Light samples are appended in two runs per block, those to even- and
+    // to odd-numbered lights, so a traversal wave mostly holds rays toward
+    // one light (directions alike) rather than a random mix.
+    uint32_t* const counters[4] = {&fp.ctrl[b + 1].n_rays, &c->n_shadow, &c->n_shadow, &c->n_backface};
     const uint32_t stride = gridDim.x * SHADE_BLOCK;
     SliceCache slc;
     // The block steps through the queue together (block_append synchronises).
@@ -1035,18 +1041,20 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
             }
         }
         // Every thread of the block reaches the appends.
-        const bool pred[3] = {r.cont, r.shadow, r.backface};
-        uint32_t slot[3];
-        block_append<3>(sh, counters, pred, slot);
+        const bool odd = (r.light & 1u) != 0;
+        const bool pred[4] = {r.cont, r.shadow && !odd, r.shadow && odd, r.backface};
+        uint32_t slot[4];
+        block_append<4>(sh, counters, pred, slot);
         if (r.cont) {
             fp.ro[nxt][slot[0]] = make_float4(r.pnt.x, r.pnt.y, r.pnt.z, T_MIN);
             fp.rd[nxt][slot[0]] = make_float4(r.dir.x, r.dir.y, r.dir.z, T_MAX);
             fp.rpath[nxt][slot[0]] = p;
         }
         if (r.shadow) {
-            fp.so[slot[1]] = make_float4(r.pnt.x, r.pnt.y, r.pnt.z, 0.001f);
-            fp.sd[slot[1]] = make_float4(r.sh_d.x, r.sh_d.y, r.sh_d.z, r.sh_tmax);
-            fp.sl[slot[1]] = make_float4(r.sh_l.x, r.sh_l.y, r.sh_l.z, __uint_as_float(p));
+            const uint32_t ss = odd ? slot[2] : slot[1];
+            fp.so[ss] = make_float4(r.pnt.x, r.pnt.y, r.pnt.z, 0.001f);
+            fp.sd[ss] = make_float4(r.sh_d.x, r.sh_d.y, r.sh_d.z, r.sh_tmax);
+            fp.sl[ss] = make_float4(r.sh_l.x, r.sh_l.y, r.sh_l.z, __uint_as_float(p));
         }
     }
 }
