@@ -56,8 +56,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s (1 GPU); equal-time relMSE guided vs plain NEE, 1080p 1M tris"
 HBM_PEAK_GBS = 8000.0
-# The traversal stage's kernel, as rocprofv3 names its dispatches.
-EXTEND_KERNEL = "ptd::k_extend<"
+# The traversal stage's kernels, as rocprofv3 names their dispatches: bounce
+# 0's camera rays walked as 64-ray packets, then one k_extend per bounce.
+EXTEND_KERNEL = ("ptd::k_camera_packet<", "ptd::k_extend<")
 COPY_KERNEL = "copyBuffer"
 SIMDS_PER_CU = 4
 
@@ -120,10 +121,11 @@ def counter_passes(args, seconds=150):
 
 
 def kernel_sum(counts, counter, pattern):
-    """(sum, dispatches) of a counter over the kernels whose name holds pattern."""
+    """(sum, dispatches) of a counter over the kernels whose name holds pattern (or one of a tuple of them)."""
+    pats = (pattern,) if isinstance(pattern, str) else pattern
     total, n = 0.0, 0
     for kern, (v, k) in counts.get(counter, {}).items():
-        if pattern in kern:
+        if any(p in kern for p in pats):
             total += v
             n += k
     return total, n
@@ -677,7 +679,8 @@ def main():
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_extend (traversal stage: closest-hit rays + the previous bounce's shadow rays)",
+            "kernel": "traversal stage: k_camera_packet (bounce 0's camera rays, 64-ray packets) and k_extend "
+                      "(closest-hit rays + the previous bounce's shadow rays)",
             "launches": launches, "avg_launch_ms": round(launch_s * 1e3, 4),
             "algorithmic_bytes_per_launch": int(algo),
             "algorithmic_model": f"{pt.QUERY_BYTES} B per query (ray in, hit out) + the scene's nodes, leaf records "

@@ -261,8 +261,15 @@ enum {
     PT_RENDER_NO_TAIL = 32,      /* every bounce stage by stage: no tail launch for the last, small queues */
     PT_RENDER_GUIDE = 64,        /* diffuse bounces sample the attached guide (pt_scene_set_guide) 50/50
                                     with the cosine lobe, one-sample MIS */
-    PT_RENDER_TRAIN = 128        /* also record each path's diffuse vertices and, after every batch, take
+    PT_RENDER_TRAIN = 128,       /* also record each path's diffuse vertices and, after every batch, take
                                     Adam steps on the guide over them (implies PT_RENDER_GUIDE) */
+    PT_RENDER_PER_RAY = 256      /* camera rays walk the tree one per lane, in each ray's own nearest-first
+                                    order (renderer.hpp's per-pixel loop). Without it, camera rays of a
+                                    scene without instances walk it 64 at a time as packets: the same
+                                    nearest hit except where two candidates tie at one t (an edge shared
+                                    by two leaves), where the packet's order picks one; a packet is 64
+                                    consecutive paths of a batch, so ties can resolve differently when
+                                    samples_per_batch or the row band changes */
 };
 
 /* Render statistics. */

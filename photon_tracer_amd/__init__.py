@@ -39,6 +39,7 @@ RENDER_DEVICE_OUTPUT = 16
 RENDER_NO_TAIL = 32
 RENDER_GUIDE = 64
 RENDER_TRAIN = 128
+RENDER_PER_RAY = 256
 GUIDE_PARAMS = 18720
 GUIDE_OUTPUTS = 32
 JOB_CAMERA = 0x100
@@ -426,17 +427,18 @@ class Scene:
                first_sample: int = 0, rows: tuple[int, int] | None = None, nee: bool = True, aux: bool = False,
                graph: bool = True, timing: bool = False, count: bool = False, samples_per_batch: int = 0,
                stack_entries: int = 0, device_out: "DeviceBuffer | None" = None, tail: bool = True,
-               guide: bool = False, train: bool = False):
+               guide: bool = False, train: bool = False, per_ray: bool = False):
         """Renders rows [j0, j1) (from the bottom; default all). Returns (rgb, aux, stats) with rgb of shape
         (rows, width, 3), the band's top row first, as render_row_linear writes rows. guide / train: sample
-        (and train) the attached guide at diffuse bounces."""
+        (and train) the attached guide at diffuse bounces. per_ray: camera rays walk one per lane in their own
+        order, not as 64-ray packets (which pick another of two tied hits now and then)."""
         j0, j1 = rows if rows else (0, height)
         p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, j0, j1,
                           (0 if nee else RENDER_NO_NEE) | (0 if graph else RENDER_NO_GRAPH) |
                           (RENDER_TIMING if timing else 0) | (RENDER_COUNT if count else 0) |
                           (RENDER_DEVICE_OUTPUT if device_out is not None else 0) |
                           (0 if tail else RENDER_NO_TAIL) | (RENDER_GUIDE if guide else 0) |
-                          (RENDER_TRAIN if train else 0), samples_per_batch,
+                          (RENDER_TRAIN if train else 0) | (RENDER_PER_RAY if per_ray else 0), samples_per_batch,
                           stack_entries)
         nrows = j1 - j0
         st = _RenderStats()
@@ -478,9 +480,10 @@ class Progressive:
     the running average)."""
 
     def __init__(self, scene: "Scene", width: int, height: int, spp: int, max_bounces: int = 10, seed: int = 0,
-                 first_sample: int = 0, nee: bool = True):
+                 first_sample: int = 0, nee: bool = True, per_ray: bool = False):
         self.width, self.height, self.spp = width, height, spp
-        p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, 0, 0, 0 if nee else RENDER_NO_NEE, 0, 0)
+        p = _RenderParams(width, height, spp, max_bounces, seed, first_sample, 0, 0,
+                          (0 if nee else RENDER_NO_NEE) | (RENDER_PER_RAY if per_ray else 0), 0, 0)
         h = ctypes.c_void_p()
         _check(library().pt_progressive_create(scene._h, ctypes.byref(p), ctypes.byref(h)))
         self._h = h

@@ -562,6 +562,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         // Below four rays per resident traversal lane, a queue no longer fills
         // the GPU stage by stage and the tail launch takes the paths.
         fp.tail_threshold = (P.flags & PT_RENDER_NO_TAIL) ? 0u : ptapi::tail_threshold(trace_grid, s->view.inst_root != NONE);
+        fp.camera_packets = (P.flags & PT_RENDER_PER_RAY) ? 0u : 1u;
         if (guided) {
             fp.guide_mode = training ? 2 : 1;
             fp.gv = G->view();

@@ -96,6 +96,7 @@ struct FrameParams {
     unsigned long long* visits;  // null, or seven totals the traversal stage counts into (Visits), then
                                  // the diffuse hits the guide's inference evaluated
     uint32_t tail_threshold;  // queues below this many rays go to the tail launch (0: never)
+    uint32_t camera_packets;  // 1: camera rays of a scene without instances are walked as wave packets
     // The learned guide: 0 off, 1 diffuse bounces sample it (one-sample MIS
     // with the cosine lobe), 2 they also record training vertices.
     int guide_mode;

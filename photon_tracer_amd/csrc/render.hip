@@ -436,6 +436,214 @@ __device__ __forceinline__ void extend_refill(const FrameParams& fp, const Scene
     }
 }
 
+// Camera rays of scenes without instances, walked a wave at a time as a
+// packet. A wave's 64 camera rays are 64 consecutive paths of the batch:
+// one pixel's samples, or the samples of a few neighbouring pixels, so they
+// visit nearly the same nodes. The wave keeps one stack in LDS of (node,
+// mask of the lanes whose rays entered it), reads each node's and leaf's
+// words once through the scalar cache, and every lane in the mask tests its
+// own ray against the four children or the leaf's primitives, with its own
+// shrinking t_max, in the arithmetic of the per-lane walk. Children are
+// taken nearest first by the entry distances of the first lane that entered
+// any of them. A lane leaves out a child only when its slab test fails, so
+// it tests every primitive its own walk could reach before its nearest hit
+// and ends with the same nearest hit, except where two candidates are tied:
+// two primitives at the same t (a ray through an edge shared by two leaves),
+// or one whose t is rounded below its box's entry, where the visiting order
+// decides which one the walk keeps. The packets start at multiples of 64
+// paths, so a packet's rays depend on the batch's paths only, not on the
+// launch shape. PT_RENDER_PER_RAY walks camera rays one per lane instead.
+constexpr int PK_DEPTH = 256;   // entries: the deepest stack pt_scene_create accepts
+struct PacketStack {
+    uint32_t code[PK_DEPTH];
+    uint32_t lo[PK_DEPTH], hi[PK_DEPTH];   // the lanes' mask
+};
+// A read-only global row read at a wave-uniform address: the constant
+// address space lets the compiler load it through the scalar cache.
+__device__ __forceinline__ float4 scalar_row(const void* base, size_t i) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const __attribute__((address_space(4))) v4* q = (const __attribute__((address_space(4))) v4*)(base);
+    const v4 r = q[i];
+    return make_float4(r.x, r.y, r.z, r.w);
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void packet_walk(const SceneView& sc, bool on, f3 o, f3 d, float tmin, float& tmax,
+                                            uint32_t& kind, uint32_t& prim, float& hu, float& hv, PacketStack& ps,
+                                            Visits* vis) {
+    const uint32_t lane = lane_id();
+    if (COUNT && on) ++vis->queries;
+    // Large spheres, in double, per lane (geometry.hpp:117-118, 157-160).
+    for (uint32_t k = 0; k < sc.n_large; ++k) {
+        if (COUNT && on) ++vis->large;
+        const uint32_t s = sc.large[k];
+        if (on && sphere_hit_d(sc.spheres[2 * s], o, d, tmin, tmax)) {
+            kind = 1;
+            prim = s;
+        }
+    }
+    const f3 inv = safe_rcp3(d);
+    unsigned long long mask = __ballot(on);
+    uint32_t code = sc.flat_root;
+    if (mask == 0 || code == NONE) return;
+    int top = 0;
+    while (true) {
+        code = __builtin_amdgcn_readfirstlane(code);   // uniform: its words load through the scalar cache
+        const bool lon = ((mask >> lane) & 1ull) != 0;
+        const uint32_t first = code & FIRST_MASK, count = code >> COUNT_SHIFT;
+        bool next = false;
+        if (count == 0) {
+            if (COUNT && lon) ++vis->nodes;
+            const size_t nd = static_cast<size_t>(first) * 8;
+            const float4 lx = scalar_row(sc.nodes, nd), ly = scalar_row(sc.nodes, nd + 1),
+                         lz = scalar_row(sc.nodes, nd + 2), hx = scalar_row(sc.nodes, nd + 3),
+                         hy = scalar_row(sc.nodes, nd + 4), hz = scalar_row(sc.nodes, nd + 5),
+                         cw = scalar_row(sc.nodes, nd + 6);
+            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w},
+                        lza[4] = {lz.x, lz.y, lz.z, lz.w};
+            const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w},
+                        hza[4] = {hz.x, hz.y, hz.z, hz.w};
+            const uint32_t cd[4] = {__float_as_uint(cw.x), __float_as_uint(cw.y), __float_as_uint(cw.z),
+                                    __float_as_uint(cw.w)};
+            unsigned long long m[4];
+            float tn[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                // visit_node's slab test, in the same operation order
+                const float x0 = (lxa[c] - o.x) * inv.x, x1 = (hxa[c] - o.x) * inv.x;
+                const float y0 = (lya[c] - o.y) * inv.y, y1 = (hya[c] - o.y) * inv.y;
+                const float z0 = (lza[c] - o.z) * inv.z, z1 = (hza[c] - o.z) * inv.z;
+                const float near = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+                const float far = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+                tn[c] = near;
+                m[c] = __ballot(lon && near <= far);
+            }
+            const unsigned long long any = m[0] | m[1] | m[2] | m[3];
+            if (any != 0) {
+                // The order: the entry distances of the first lane that entered a child.
+                const int leader = __builtin_ctzll(any);
+                float lt[4];
+                int nv = 0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float t = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(tn[c]), leader));
+                    lt[c] = ((m[c] >> leader) & 1ull) ? t : __builtin_inff();
+                    nv += m[c] != 0 ? 1 : 0;
+                }
+                // rank 0: taken next; the others pushed so the nearer pop first
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (m[c] == 0) continue;
+                    int rank = 0;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (e != c && m[e] != 0 && (lt[e] < lt[c] || (lt[e] == lt[c] && e < c))) ++rank;
+                    if (rank == 0) {
+                        code = cd[c];
+                        mask = m[c];
+                        next = true;
+                    } else {
+                        const int at = top + (nv - 1 - rank);
+                        if (lane == 0) {
+                            ps.code[at] = cd[c];
+                            ps.lo[at] = static_cast<uint32_t>(m[c]);
+                            ps.hi[at] = static_cast<uint32_t>(m[c] >> 32);
+                        }
+                    }
+                }
+                top += nv - 1;
+            }
+        } else {
+            // A leaf's primitives in order (geometry.hpp:119-127).
+            for (uint32_t i = first; i < first + count; ++i) {
+                const float4 r0 = scalar_row(sc.leaf, 3 * size_t(i)), r1 = scalar_row(sc.leaf, 3 * size_t(i) + 1);
+                if (COUNT && lon) {
+                    ++vis->leafrefs;
+                    if (__float_as_uint(r1.w) != 0u) ++vis->spheres;
+                    else ++vis->tris;
+                }
+                if (__float_as_uint(r1.w) != 0u) {
+                    if (lon && sphere_hit_f(make_float4(r0.x, r0.y, r0.z, r1.x), o, d, tmin, tmax)) {
+                        kind = 2;
+                        prim = __float_as_uint(r0.w);
+                    }
+                } else {
+                    const float4 r2 = scalar_row(sc.leaf, 3 * size_t(i) + 2);
+                    if (lon && tri_hit(xyz(r0), xyz(r1), xyz(r2), o, d, tmin, tmax, hu, hv)) {
+                        kind = 3;
+                        prim = __float_as_uint(r0.w);
+                    }
+                }
+            }
+        }
+        if (!next) {
+            if (top == 0) break;
+            --top;
+            __builtin_amdgcn_wave_barrier();
+            code = __builtin_amdgcn_readfirstlane(ps.code[top]);
+            const uint32_t lo = __builtin_amdgcn_readfirstlane(ps.lo[top]), hi = __builtin_amdgcn_readfirstlane(ps.hi[top]);
+            mask = (static_cast<unsigned long long>(hi) << 32) | lo;
+        }
+    }
+}
+
+// The camera-ray launch with packet walks: the waves take runs of the frame's
+// paths (XCD segments as extend_refill's) and trace 64 at a time.
+template <bool COUNT>
+__device__ __forceinline__ void extend_packet(const FrameParams& fp, const SceneView& sc, Ctrl* c, uint32_t n,
+                                              PacketStack& ps, Visits& vis) {
+    const FrameConst fc = *fp.frame;
+    const uint32_t chunk = 64 * fetch_groups(n);
+    const uint32_t nseg = uint64_t(n) >= uint64_t(SEGMENT_MIN) * gridDim.x * blockDim.x ? XCD_SEGMENTS : 1u;
+    uint32_t seg = blockIdx.x % nseg, seg_done = 0;
+    while (true) {
+        // Segment bounds rounded to multiples of 64 paths (the last ends at n).
+        const uint32_t s_lo = static_cast<uint32_t>(uint64_t(n) * seg / nseg) & ~63u;
+        const uint32_t s_hi = seg + 1 == nseg ? n : static_cast<uint32_t>(uint64_t(n) * (seg + 1) / nseg) & ~63u;
+        const uint32_t s_len = s_hi - s_lo;
+        const uint32_t got = wave_fetch(&c->w_seg[seg], chunk);
+        if (got >= s_len) {
+            if (++seg_done == nseg) break;
+            seg = (seg + 1) % nseg;
+            continue;
+        }
+        const uint32_t run_end = s_lo + min(got + chunk, s_len);
+        for (uint32_t base = s_lo + got; base < run_end; base += 64) {
+            const uint32_t item = base + lane_id();
+            const bool on = item < run_end;
+            f3 o = mk(0, 0, 0), d = mk(0, 0, 1);
+            if (on) {
+                const CameraSample cs = camera_sample(fp, fc, item);
+                o = cs.o;
+                d = cs.d;
+            }
+            float tmax = T_MAX, hu = 0, hv = 0;
+            uint32_t kind = 0, prim = 0;
+            packet_walk<COUNT>(sc, on, o, d, T_MIN, tmax, kind, prim, hu, hv, ps, &vis);
+            if (on) {
+                Hit h;
+                h.t = tmax;
+                h.u = hu;
+                h.v = hv;
+                h.kind = kind;
+                h.prim = prim;
+                h.inst = NONE;
+                fp.hits[item] = h;
+            }
+        }
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_camera_packet(FrameParams fp, int b) {
+    __shared__ PacketStack stk[TRACE_BLOCK / 64];
+    Visits vis;
+    Ctrl* c = fp.ctrl + b;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->n_rays = fp.P;
+    extend_packet<COUNT>(fp, fp.sc, c, fp.P, stk[threadIdx.x >> 6], vis);
+    if (COUNT) add_visits(fp.visits, vis);
+}
+
 template <int S, bool COUNT, bool INST, bool PRIMARY, bool POOL = false>
 __global__ __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void k_extend(FrameParams fp, int b) {
@@ -1404,8 +1612,14 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
         }
         if (ev) (void)hipEventRecord(ev[2 * b], stream);
         if (b == 0) {
-            if (fp.visits) k_extend<S, true, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
-            else k_extend<S, false, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            if (!INST && fp.camera_packets) {
+                if (fp.visits) k_camera_packet<true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+                else k_camera_packet<false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            } else if (fp.visits) {
+                k_extend<S, true, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            } else {
+                k_extend<S, false, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            }
         } else {
             if (fp.visits) k_extend<S, true, INST, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
             else k_extend<S, false, INST, false><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);

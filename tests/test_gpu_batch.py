@@ -1,7 +1,8 @@
 """Batched renders (pt_render_batch): many jobs -- different scenes, views,
 image sizes, spp, seeds -- in one call over one merged device scene. Every
 job's image must be bit for bit the image pt_render gives for that job alone
-(which the oracle and the reference fixtures pin), in shared-queue passes
+with camera rays walked per ray (which the oracle and the reference fixtures
+pin), in shared-queue passes
 (slices of many jobs in one bounce loop) or per-job loops over streams,
 whatever the pass or batch sizes, streams and launch shape, with or without
 the captured graphs, and after the autotuner."""
@@ -50,7 +51,7 @@ def single(scenes, j, depth=10):
     ptgp = scenes[j.scene] if j.camera is None else with_camera(scenes[j.scene], j.camera)
     with pt.Scene(ptgp) as s:
         img, _, st = s.render(j.width, j.height, j.spp, max_bounces=depth, seed=j.seed, first_sample=j.first_sample,
-                              nee=j.nee)
+                              nee=j.nee, per_ray=True)
     return img, st
 
 

@@ -67,12 +67,13 @@ def test_guided_render_is_deterministic():
         a, _, sa = s.render(80, 60, 8, seed=3, guide=True)
         b, _, sb = s.render(80, 60, 8, seed=3, guide=True)
         c, _, _ = s.render(80, 60, 8, seed=3, guide=True, graph=False)
-        d, _, _ = s.render(80, 60, 8, seed=3, guide=True, samples_per_batch=2)
         e, _, _ = s.render(80, 60, 8, seed=3)
+        r, _, _ = s.render(80, 60, 8, seed=3, guide=True, per_ray=True)
+        d, _, _ = s.render(80, 60, 8, seed=3, guide=True, samples_per_batch=2, per_ray=True)
     assert np.array_equal(a, b) and sa.rays == sb.rays
     assert np.array_equal(a, c)     # graph replay == eager
-    assert np.array_equal(a, d)     # batch size does not matter
     assert not np.array_equal(a, e)
+    assert np.array_equal(r, d)     # camera rays walked per ray: batch size does not matter
 
 
 def test_guide_stats():
