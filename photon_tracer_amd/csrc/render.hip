@@ -516,7 +516,7 @@ __device__ __forceinline__ void packet_walk(const SceneView& sc, bool on, f3 o, 
                 const float near = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
                 const float far = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
                 tn[c] = near;
-                m[c] = __ballot(lon && near <= far);
+                m[c] = mask & __ballot(near <= far);   // lanes in the packet whose slab test passes
             }
             const unsigned long long any = m[0] | m[1] | m[2] | m[3];
             if (any != 0) {
