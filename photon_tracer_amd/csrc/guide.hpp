@@ -65,12 +65,24 @@ __device__ __forceinline__ float clamp01(float v) { return fminf(fmaxf(v, 0.0f),
 // through.
 constexpr float BLOB_K = 0.8493218002880191f;   // sqrt(log2(e) / 2)
 
-__device__ __forceinline__ float blob_scaled(float vk, float ck) {
-    const float d = vk - ck;
-    return __builtin_amdgcn_exp2f(-(d * d));
-}
-__device__ __forceinline__ float blob(float v, int bins, int i) {
-    return blob_scaled(v * (static_cast<float>(bins) * BLOB_K), (static_cast<float>(i) + 0.5f) * BLOB_K);
+// Eight neighbouring blobs, exp2(-(vk - c_j)^2) for c_j = c0 + j BLOB_K, from
+// two exponentials: with d_j = vk - c_j, d_j^2 - d_{j+1}^2 = K (2 d_j - K), so
+// b_{j+1} = b_j r_j with r_j = exp2(K (2 d_j - K)) and r_{j+1} = r_j
+// exp2(-2 K^2) = r_j / e (2 K^2 = log2 e). Each product adds half an ulp: the
+// run is within 14 ulps of the direct values, far below the FP16 rounding
+// they go through. On the encoding's ranges (vk in [0, 16 K], c0 a bin
+// centre) r_0 lies in [2^-13, 2^22]; where b_0 underflows, every value of the
+// run is below 2^-39, zero in FP16 as the direct values are.
+constexpr float BLOB_Q = 0.36787944117144233f;   // exp2(-2 BLOB_K^2) = 1/e
+__device__ __forceinline__ void blob_run8(float vk, float c0, float (&b)[8]) {
+    const float d0 = vk - c0;
+    b[0] = __builtin_amdgcn_exp2f(-(d0 * d0));
+    float r = __builtin_amdgcn_exp2f(BLOB_K * (2.0f * d0 - BLOB_K));
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        b[j] = b[j - 1] * r;
+        r = r * BLOB_Q;
+    }
 }
 
 // The normal's angles for the encoding, from polynomial approximations of
@@ -128,15 +140,21 @@ __device__ __forceinline__ void encode_scalars(const GuideView& g, f3 x, f3 n, f
 __device__ __forceinline__ void encode(const GuideView& g, f3 x, f3 n, _Float16* row) {
     float e[5];
     encode_scalars(g, x, n, e);
+    float b[8];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) row[16 * a + i] = static_cast<_Float16>(blob(e[a], 16, i));
+        for (int h = 0; h < 2; ++h) {
+            blob_run8(e[a] * (16.0f * BLOB_K), (static_cast<float>(8 * h) + 0.5f) * BLOB_K, b);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) row[16 * a + 8 * h + i] = static_cast<_Float16>(b[i]);
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        row[48 + i] = static_cast<_Float16>(blob(e[3], 8, i));
-        row[56 + i] = static_cast<_Float16>(blob(e[4], 8, i));
+    for (int a = 3; a < 5; ++a) {
+        blob_run8(e[a] * (8.0f * BLOB_K), 0.5f * BLOB_K, b);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) row[48 + 8 * (a - 3) + i] = static_cast<_Float16>(b[i]);
     }
 }
 
@@ -545,24 +563,20 @@ typedef h8 RegActs[2][4];
 // q1 of query 32 + c.
 __device__ __forceinline__ void encode_blobs(const float (&q0)[5], const float (&q1)[5], RegActs& X) {
     const int kh = lane_kh();
-    float c16[8], c8[8];   // bin centres of this lane's half, scaled
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        c16[j] = (static_cast<float>(8 * kh + j) + 0.5f) * BLOB_K;
-        c8[j] = (static_cast<float>(j) + 0.5f) * BLOB_K;
-    }
+    const float c16 = (static_cast<float>(8 * kh) + 0.5f) * BLOB_K;   // first bin centre of this lane's half
+    float b[8];
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
         const float(&q)[5] = nb == 0 ? q0 : q1;
 #pragma unroll
         for (int kb = 0; kb < 3; ++kb) {
-            const float vk = q[kb] * (16.0f * BLOB_K);
+            blob_run8(q[kb] * (16.0f * BLOB_K), c16, b);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) X[nb][kb][j] = static_cast<_Float16>(blob_scaled(vk, c16[j]));
+            for (int j = 0; j < 8; ++j) X[nb][kb][j] = static_cast<_Float16>(b[j]);
         }
-        const float vk = (kh ? q[4] : q[3]) * (8.0f * BLOB_K);
+        blob_run8((kh ? q[4] : q[3]) * (8.0f * BLOB_K), 0.5f * BLOB_K, b);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) X[nb][3][j] = static_cast<_Float16>(blob_scaled(vk, c8[j]));
+        for (int j = 0; j < 8; ++j) X[nb][3][j] = static_cast<_Float16>(b[j]);
     }
 }
 
