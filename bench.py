@@ -376,10 +376,13 @@ def guided_leg(pt, ref_spp, spp_plain=1024, steps_per_batch=128, guide_seeds=(2,
                 n += 1
             steps, used, loss = g.stats()
             # The guided render's cost per spp, measured at the batch size the
-            # plain render ran at (the library's choice for this frame), then
-            # the rest of the budget in whole batches of that size, so both
-            # tracers run the same launch shapes.
+            # plain render ran at (the library's choice for this frame) once
+            # that shape's graph is captured (the capture's host time falls
+            # inside the first render's device-time events), then the rest of
+            # the budget in whole batches of that size, so both tracers run
+            # the same launch shapes.
             spb = max(16, sp.samples_per_batch)
+            s.render(W, H, spb, seed=6, guide=True)   # captures the guided graph of this batch shape
             _, _, st1 = s.render(W, H, spb, seed=7, guide=True)
             per_spp = st1.render_ms / spb
             spp_g = max(spb, int((budget - train_ms) / per_spp) // spb * spb)

@@ -597,6 +597,34 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         }
 
         const int batches = P.spp / spb;
+        const bool timing = stats && (P.flags & PT_RENDER_TIMING);
+        if (!timing && !training && !(P.flags & PT_RENDER_NO_GRAPH) && !counting) {
+            // The batch's whole bounce loop as one graph, replayed per batch
+            // and kept for the next render of the same shape; kernels read
+            // queue sizes, the seed and the batch number from HBM. Captured
+            // before the render's first event, so render_ms is device time
+            // and not the host's capture.
+            auto& g = s->graph;
+            const bool reuse = g.exec && g.S == S && g.trace_grid == trace_grid && g.shade_grid == shade_grid &&
+                               std::memcmp(&g.key, &fp, sizeof fp) == 0;
+            if (!reuse) {
+                s->drop_graph();
+                hipGraph_t graph;
+                HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                const hipError_t le = ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr);
+                const hipError_t ce = hipStreamEndCapture(s->stream, &graph);
+                HIPC(le);
+                HIPC(ce);
+                const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) g.exec = nullptr;
+                HIPC(ie);
+                g.key = fp;
+                g.S = S;
+                g.trace_grid = trace_grid;
+                g.shade_grid = shade_grid;
+            }
+        }
         hipEvent_t e0, e1;
         HIPC(hipEventCreate(&e0));
         HIPC(hipEventCreate(&e1));
@@ -605,7 +633,6 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
         HIPC(ptd::launch_clear_frame(fp, s->stream));
         HIPC(hipEventRecord(e0, s->stream));
         double extend_ms = 0, shadow_ms = 0, guide_ms = 0;
-        const bool timing = stats && (P.flags & PT_RENDER_TIMING);
         if (timing) {
             // Eager, with an event pair around every traversal launch and,
             // in guided renders, every inference launch (pairs 2 depth ..).
@@ -642,30 +669,7 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             for (int b = 0; b < batches; ++b)
                 HIPC(ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr));
         } else {
-            // The batch's whole bounce loop as one graph, replayed per batch
-            // and kept for the next render of the same shape; kernels read
-            // queue sizes, the seed and the batch number from HBM.
-            auto& g = s->graph;
-            const bool reuse = g.exec && g.S == S && g.trace_grid == trace_grid && g.shade_grid == shade_grid &&
-                               std::memcmp(&g.key, &fp, sizeof fp) == 0;
-            if (!reuse) {
-                s->drop_graph();
-                hipGraph_t graph;
-                HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                const hipError_t le = ptd::launch_batch(fp, S, trace_grid, shade_grid, s->stream, nullptr);
-                const hipError_t ce = hipStreamEndCapture(s->stream, &graph);
-                HIPC(le);
-                HIPC(ce);
-                const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                if (ie != hipSuccess) g.exec = nullptr;
-                HIPC(ie);
-                g.key = fp;
-                g.S = S;
-                g.trace_grid = trace_grid;
-                g.shade_grid = shade_grid;
-            }
-            for (int b = 0; b < batches; ++b) HIPC(hipGraphLaunch(g.exec, s->stream));
+            for (int b = 0; b < batches; ++b) HIPC(hipGraphLaunch(s->graph.exec, s->stream));
         }
         HIPC(hipEventRecord(e1, s->stream));
         HIPC(hipEventSynchronize(e1));
