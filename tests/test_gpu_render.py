@@ -67,12 +67,12 @@ def test_tail_launch_equals_stage_by_stage(name):
 
 @pytest.mark.parametrize("name,scale,size", [("field", None, (192, 108, 16)), ("room", None, (192, 108, 16)),
                                              ("materials", None, (192, 108, 16)), ("config0", None, (192, 108, 16)),
-                                             ("config1", 1.0, (480, 270, 16))])
+                                             ("config1", 1.0, (480, 270, 16)), ("teapot", None, (192, 108, 16))])
 def test_camera_packets_agree_with_per_ray_walk(name, scale, size):
     """Camera rays walked 64 at a time as packets (the default for scenes without instances) find each ray's
     own nearest hit; only a tie, two candidates at one t, can go the other way. The images then agree bit for
     bit on at least 99.99% of pixels (the tie bound for primitive ids) and the ray counts to 1e-4. config1 at
-    scale 1 is the 1,003,520-triangle mesh."""
+    scale 1 is the 1,003,520-triangle mesh, teapot the reference's teapot mesh."""
     ptgp = pt.pack(pt.generate(1, 1, scale)) if scale else scene_of(name)
     W, H, spp = size
     with pt.Scene(ptgp) as s:
