@@ -511,9 +511,12 @@ pt_status pt_render_batch(pt_batch* b, const pt_batch_job* jobs, int n_jobs, con
         pt_scene* s = b->scene;
         if (P.max_bounces < 1 || P.max_bounces >= ptd::MAX_BOUNCES)
             throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: max_bounces must be in [1, 63]");
-        if (P.flags & ~(PT_RENDER_NO_GRAPH | PT_RENDER_DEVICE_OUTPUT | PT_RENDER_NO_TAIL | PT_BATCH_STREAMS))
-            throw pt::status_error(PT_ERR_INVALID,
-                                   "pt_render_batch: flags other than NO_GRAPH, DEVICE_OUTPUT, NO_TAIL, BATCH_STREAMS");
+        // PT_RENDER_PER_RAY is accepted and changes nothing: batched renders
+        // always walk camera rays one per lane.
+        if (P.flags & ~(PT_RENDER_NO_GRAPH | PT_RENDER_DEVICE_OUTPUT | PT_RENDER_NO_TAIL | PT_BATCH_STREAMS |
+                        PT_RENDER_PER_RAY))
+            throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: flags other than NO_GRAPH, DEVICE_OUTPUT, "
+                                                   "NO_TAIL, BATCH_STREAMS, PER_RAY");
         check_jobs(b, jobs, n_jobs);
         const render_shape sh = shape_for(b, P, jobs, n_jobs);
         if (!(P.flags & PT_BATCH_STREAMS)) {
