@@ -399,8 +399,9 @@ enum { PT_BATCH_STREAMS = 0x200 };
 
 typedef struct {
     int32_t max_bounces;        /* every job's bounce cap, 1 .. 63 */
-    int32_t flags;              /* PT_RENDER_NO_GRAPH, PT_RENDER_DEVICE_OUTPUT, PT_RENDER_NO_TAIL, PT_BATCH_STREAMS;
-                                   PT_RENDER_PER_RAY is accepted: camera rays of a batch are always walked per ray */
+    int32_t flags;              /* PT_RENDER_NO_GRAPH, PT_RENDER_DEVICE_OUTPUT, PT_RENDER_NO_TAIL, PT_BATCH_STREAMS,
+                                   PT_RENDER_PER_RAY (camera rays walked per lane, as pt_render with the flag;
+                                   without it they are walked as 64-ray packets per job) */
     int32_t stack_entries;      /* 8, 16 or 32; 0 = tuned (pt_batch_autotune) or 8 */
     int32_t blocks_per_cu;      /* traversal blocks per CU; 0 = tuned or the occupancy limit */
     int32_t streams;            /* HIP streams the jobs are spread over, 1 .. 8; 0 = tuned or 2 */

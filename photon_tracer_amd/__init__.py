@@ -757,13 +757,15 @@ class Batch:
 
     def render(self, jobs, max_bounces: int = 10, graph: bool = True, tail: bool = True, stack_entries: int = 0,
                blocks_per_cu: int = 0, streams: int = 0, max_paths: int = 0, device_out: "DeviceBuffer | None" = None,
-               shared: bool = True):
+               shared: bool = True, per_ray: bool = False):
         """Renders every job. Returns (images, stats): one (height, width, 3) array a job, top row first
         (None with device_out: the images stay in that device buffer, one after another). shared: the jobs'
-        paths in shared-queue passes; False: each job's own bounce loops over `streams` HIP streams."""
+        paths in shared-queue passes; False: each job's own bounce loops over `streams` HIP streams. per_ray:
+        camera rays walked one per lane (each job's image then equals pt_render's with per_ray bit for bit)."""
         arr = _job_array(jobs)
         flags = (0 if graph else RENDER_NO_GRAPH) | (0 if tail else RENDER_NO_TAIL) | \
-            (RENDER_DEVICE_OUTPUT if device_out is not None else 0) | (0 if shared else BATCH_STREAMS)
+            (RENDER_DEVICE_OUTPUT if device_out is not None else 0) | (0 if shared else BATCH_STREAMS) | \
+            (RENDER_PER_RAY if per_ray else 0)
         p = _BatchParams(int(max_bounces), flags, int(stack_entries), int(blocks_per_cu), int(streams), 0,
                          int(max_paths))
         sizes = [int(j.width) * int(j.height) for j in jobs]

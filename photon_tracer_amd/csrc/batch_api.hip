@@ -366,6 +366,7 @@ void render_pool(pt_batch* b, const pt_batch_job* jobs, int n_jobs, const pt_bat
     base.totals = b->totals.as<unsigned long long>();
     base.spill = st.spill.as<uint2>();
     base.tail_threshold = (P.flags & PT_RENDER_NO_TAIL) ? 0u : ptapi::tail_threshold(trace_grid, false);
+    base.camera_packets = (P.flags & PT_RENDER_PER_RAY) ? 0u : 1u;
     // The launches: clear every sum, the passes, each job's resolve.
     std::vector<ptd::FrameParams> launches;
     ptd::FrameParams clear = base;
@@ -511,8 +512,6 @@ pt_status pt_render_batch(pt_batch* b, const pt_batch_job* jobs, int n_jobs, con
         pt_scene* s = b->scene;
         if (P.max_bounces < 1 || P.max_bounces >= ptd::MAX_BOUNCES)
             throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: max_bounces must be in [1, 63]");
-        // PT_RENDER_PER_RAY is accepted and changes nothing: batched renders
-        // always walk camera rays one per lane.
         if (P.flags & ~(PT_RENDER_NO_GRAPH | PT_RENDER_DEVICE_OUTPUT | PT_RENDER_NO_TAIL | PT_BATCH_STREAMS |
                         PT_RENDER_PER_RAY))
             throw pt::status_error(PT_ERR_INVALID, "pt_render_batch: flags other than NO_GRAPH, DEVICE_OUTPUT, "
@@ -623,6 +622,7 @@ pt_status pt_render_batch(pt_batch* b, const pt_batch_job* jobs, int n_jobs, con
             fp.spill = st.spill.as<uint2>();
             fp.tail_threshold =
                 (P.flags & PT_RENDER_NO_TAIL) ? 0u : ptapi::tail_threshold(trace_grid, false);   // batched scenes are flat
+            fp.camera_packets = (P.flags & PT_RENDER_PER_RAY) ? 0u : 1u;
             return fp;
         };
         // A stream's whole sequence: for each of its jobs, clear, the

@@ -644,6 +644,66 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_camera_packet(FrameParams fp, i
     if (COUNT) add_visits(fp.visits, vis);
 }
 
+// The camera rays of a shared-queue pass as packets: a run of 64 paths can
+// hold the paths of more than one job's slice, so the run is walked once per
+// slice among its lanes, each time as a packet of that slice's lanes over the
+// job's tree (slice_scene), its rays from its camera (camera_sample_pool).
+__global__ __launch_bounds__(TRACE_BLOCK) void k_camera_packet_pool(FrameParams fp, int b) {
+    __shared__ PacketStack stk[TRACE_BLOCK / 64];
+    PacketStack& ps = stk[threadIdx.x >> 6];
+    Visits vis;
+    Ctrl* c = fp.ctrl + b;
+    const uint32_t n = fp.P;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->n_rays = n;
+    const uint32_t chunk = 64 * fetch_groups(n);
+    const uint32_t nseg = uint64_t(n) >= uint64_t(SEGMENT_MIN) * gridDim.x * blockDim.x ? XCD_SEGMENTS : 1u;
+    uint32_t seg = blockIdx.x % nseg, seg_done = 0;
+    SliceCache slc;
+    while (true) {
+        const uint32_t s_lo = static_cast<uint32_t>(uint64_t(n) * seg / nseg) & ~63u;
+        const uint32_t s_hi = seg + 1 == nseg ? n : static_cast<uint32_t>(uint64_t(n) * (seg + 1) / nseg) & ~63u;
+        const uint32_t s_len = s_hi - s_lo;
+        const uint32_t got = wave_fetch(&c->w_seg[seg], chunk);
+        if (got >= s_len) {
+            if (++seg_done == nseg) break;
+            seg = (seg + 1) % nseg;
+            continue;
+        }
+        const uint32_t run_end = s_lo + min(got + chunk, s_len);
+        for (uint32_t base = s_lo + got; base < run_end; base += 64) {
+            const uint32_t item = base + lane_id();
+            const bool on = item < run_end;
+            const uint32_t k = on ? slc.find(fp, item) : 0u;
+            unsigned long long todo = __ballot(on);
+            while (todo != 0) {
+                const uint32_t k0 = __builtin_amdgcn_readlane(k, __builtin_ctzll(todo));
+                const bool mine = on && k == k0;
+                todo &= ~__ballot(mine);
+                const PoolSlice sl = fp.slices[k0];
+                f3 o = mk(0, 0, 0), d = mk(0, 0, 1);
+                if (mine) {
+                    const CameraSample cs = camera_sample_pool(sl, item - slc.lo);
+                    o = cs.o;
+                    d = cs.d;
+                }
+                float tmax = T_MAX, hu = 0, hv = 0;
+                uint32_t kind = 0, prim = 0;
+                packet_walk<false>(slice_scene(fp.sc, sl), mine, o, d, T_MIN, tmax, kind, prim, hu, hv, ps, &vis);
+                if (mine) {
+                    Hit h;
+                    h.t = tmax;
+                    h.u = hu;
+                    h.v = hv;
+                    h.kind = kind;
+                    h.prim = prim;
+                    h.inst = NONE;
+                    fp.hits[item] = h;
+                }
+            }
+        }
+    }
+}
+
 template <int S, bool COUNT, bool INST, bool PRIMARY, bool POOL = false>
 __global__ __launch_bounds__(TRACE_BLOCK)
 __attribute__((amdgpu_waves_per_eu(trace_occupancy<S, INST, true>::waves))) void k_extend(FrameParams fp, int b) {
@@ -1791,7 +1851,8 @@ static void launch_pool_loop(const FrameParams& fp, int trace_grid, int shade_gr
             k_extend<S, false, false, false, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
             k_shade<false, false, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
         } else {
-            k_extend<S, false, false, true, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            if (fp.camera_packets) k_camera_packet_pool<<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            else k_extend<S, false, false, true, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
             k_shade<true, false, true><<<shade_grid, SHADE_BLOCK, 0, stream>>>(fp, b);
         }
     }

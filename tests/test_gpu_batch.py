@@ -67,7 +67,7 @@ def test_batch_equals_single_renders(singles, shared):
     """Shared-queue passes (the default) and per-job bounce loops over streams."""
     sc, jobs, ref = singles
     with pt.Batch(sc) as b:
-        imgs, st = b.render(jobs, shared=shared)
+        imgs, st = b.render(jobs, shared=shared, per_ray=True)
     for k, (img, (want, _)) in enumerate(zip(imgs, ref)):
         assert img.shape == want.shape
         same = np.array_equal(img.view(np.uint32), want.view(np.uint32))
@@ -86,8 +86,8 @@ def test_batch_sizes_and_streams_change_nothing(singles, max_paths, streams, sha
     between them (shared queues); jobs spread over several streams; eager launches and the graphs agree."""
     sc, jobs, ref = singles
     with pt.Batch(sc) as b:
-        a, sa = b.render(jobs, max_paths=max_paths, streams=streams, shared=shared)
-        e, se = b.render(jobs, max_paths=max_paths, streams=streams, graph=False, shared=shared)
+        a, sa = b.render(jobs, max_paths=max_paths, streams=streams, shared=shared, per_ray=True)
+        e, se = b.render(jobs, max_paths=max_paths, streams=streams, graph=False, shared=shared, per_ray=True)
     assert sa.batches == se.batches
     if max_paths == 6000:
         assert sa.batches > 3 * len(jobs) if not shared else sa.batches > 40
@@ -104,9 +104,9 @@ def test_device_output_and_stage_by_stage(singles, shared):
     n = sum(j.width * j.height for j in jobs)
     buf = pt.DeviceBuffer(12 * n)
     with pt.Batch(sc) as b:
-        _, sd = b.render(jobs, device_out=buf, shared=shared)
+        _, sd = b.render(jobs, device_out=buf, shared=shared, per_ray=True)
         flat = buf.download(np.zeros(3 * n, np.float32))
-        nt, _ = b.render(jobs, tail=False, shared=shared)
+        nt, _ = b.render(jobs, tail=False, shared=shared, per_ray=True)
     at = 0
     for j, x, (want, _) in zip(jobs, nt, ref):
         k = j.width * j.height
@@ -120,14 +120,30 @@ def test_launch_shapes_change_nothing(singles):
     sc, jobs, ref = singles
     with pt.Batch(sc) as b:
         for S, per_cu, shared in ((16, 0, True), (32, 2, True), (8, 3, True), (16, 0, False), (8, 3, False)):
-            imgs, _ = b.render(jobs, stack_entries=S, blocks_per_cu=per_cu, shared=shared)
+            imgs, _ = b.render(jobs, stack_entries=S, blocks_per_cu=per_cu, shared=shared, per_ray=True)
             for x, (want, _) in zip(imgs, ref):
                 assert np.array_equal(x, want), (S, per_cu, shared)
         # the two modes one after the other on the same batch (each keeps its own captured graph)
         for shared in (True, False, True):
-            imgs, _ = b.render(jobs, shared=shared)
+            imgs, _ = b.render(jobs, shared=shared, per_ray=True)
             for x, (want, _) in zip(imgs, ref):
                 assert np.array_equal(x, want), shared
+
+
+@pytest.mark.parametrize("shared", [True, False])
+def test_batch_camera_packets_agree_with_per_ray(singles, shared):
+    """Batched camera rays walked as 64-ray packets (the default; a run of the shared queue holding paths of
+    two jobs is walked once per job) against the same batch walked per ray: each job's image bit for bit
+    equal on at least 99.99% of its pixels (the tie bound), the ray counts to 1e-4."""
+    sc, jobs, _ = singles
+    with pt.Batch(sc) as b:
+        pk, sp = b.render(jobs, shared=shared)
+        pr, sr = b.render(jobs, shared=shared, per_ray=True)
+    for k, (x, y) in enumerate(zip(pk, pr)):
+        same = np.all(x.view(np.uint32) == y.view(np.uint32), axis=2)
+        print(f"job {k}: {int((~same).sum())} of {same.size} pixels differ", flush=True)
+        assert same.mean() >= 0.9999, k
+    assert abs(sp.rays - sr.rays) <= 1e-4 * sr.rays and sp.primary_rays == sr.primary_rays
 
 
 def test_batch_job_matches_oracle():
@@ -150,7 +166,7 @@ def test_autotune_then_render(singles):
               f"tried {[(t[0], t[1], t[2], t[3], round(t[4])) for t in r.tried]}", flush=True)
         assert r.stack_entries in (8, 16, 32) and r.blocks_per_cu >= 1 and 1 <= r.streams <= 8
         assert len(r.tried) >= 6
-        imgs, _ = b.render(jobs)
+        imgs, _ = b.render(jobs, per_ray=True)
     for x, (want, _) in zip(imgs, ref):
         assert np.array_equal(x, want)
 
