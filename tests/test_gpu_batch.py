@@ -1,8 +1,8 @@
 """Batched renders (pt_render_batch): many jobs -- different scenes, views,
 image sizes, spp, seeds -- in one call over one merged device scene. Every
-job's image must be bit for bit the image pt_render gives for that job alone
-with camera rays walked per ray (which the oracle and the reference fixtures
-pin), in shared-queue passes
+job's image, camera rays walked per ray in both (PT_RENDER_PER_RAY), must be
+bit for bit the image pt_render gives for that job alone (which the oracle and
+the reference fixtures pin), in shared-queue passes
 (slices of many jobs in one bounce loop) or per-job loops over streams,
 whatever the pass or batch sizes, streams and launch shape, with or without
 the captured graphs, and after the autotuner."""
