@@ -144,13 +144,42 @@ __device__ __forceinline__ uint32_t slice_of(const uint32_t* base, uint32_t n, u
 
 // A lane's last slice and its path range: queues keep paths roughly in
 // path order, so the next path is mostly in the same slice.
+// A read-only word at a wave-uniform index, through the scalar cache.
+__device__ __forceinline__ uint32_t scalar_u32(const uint32_t* base, uint32_t i) {
+    const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)(base);
+    return q[i];
+}
+
+// slice_of for a wave-uniform p, by scalar loads.
+__device__ __forceinline__ uint32_t slice_of_uniform(const uint32_t* base, uint32_t n, uint32_t p) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (scalar_u32(base, mid) <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// A lane's last slice and its path range. A lookup outside the range first
+// tries the slice of the first lane that looks one up (the wave's paths are
+// mostly one slice's), found once for the wave by scalar loads, and searches
+// on its own only when its path lies outside that one too.
 struct SliceCache {
     uint32_t k = 0, lo = 1, hi = 0;   // an empty range: the first lookup searches
     __device__ __forceinline__ uint32_t find(const FrameParams& fp, uint32_t p) {
         if (!(p >= lo && p < hi)) {
-            k = slice_of(fp.slice_path, fp.n_slices, p);
-            lo = fp.slice_path[k];
-            hi = fp.slice_path[k + 1];   // the array holds n_slices + 1 entries
+            const uint32_t k0 = slice_of_uniform(fp.slice_path, fp.n_slices, __builtin_amdgcn_readfirstlane(p));
+            const uint32_t lo0 = scalar_u32(fp.slice_path, k0), hi0 = scalar_u32(fp.slice_path, k0 + 1);
+            if (p >= lo0 && p < hi0) {
+                k = k0;
+                lo = lo0;
+                hi = hi0;
+            } else {
+                k = slice_of(fp.slice_path, fp.n_slices, p);
+                lo = fp.slice_path[k];
+                hi = fp.slice_path[k + 1];   // the array holds n_slices + 1 entries
+            }
         }
         return k;
     }
