@@ -598,7 +598,15 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
 
         const int batches = P.spp / spb;
         const bool timing = stats && (P.flags & PT_RENDER_TIMING);
-        if (!timing && !training && !(P.flags & PT_RENDER_NO_GRAPH) && !counting) {
+        const bool use_graph = !timing && !training && !(P.flags & PT_RENDER_NO_GRAPH) && !counting;
+        // A graph of a shape rendered before whose tail launch took over at
+        // bounce t ends there: the tail takes over at t whatever the queue
+        // sizes (its paths come out the same), and the launches after it,
+        // which would return at once, are left out.
+        const ptd::FrameParams shape_key = fp;
+        if (use_graph && s->tail_hint != 0 && std::memcmp(&s->tail_hint_key, &shape_key, sizeof fp) == 0)
+            fp.tail_last = s->tail_hint;
+        if (use_graph) {
             // The batch's whole bounce loop as one graph, replayed per batch
             // and kept for the next render of the same shape; kernels read
             // queue sizes, the seed and the batch number from HBM. Captured
@@ -686,9 +694,15 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             HIPC(hipMemcpyAsync(rgb, B.out_rgb.p, npix * 12, hipMemcpyDeviceToHost, s->stream));
             if (want_aux) HIPC(hipMemcpyAsync(aux, B.out_aux.p, npix * 28, hipMemcpyDeviceToHost, s->stream));
         }
-        unsigned long long totals[3] = {0, 0, 0};
+        unsigned long long totals[4] = {0, 0, 0, 0};
         HIPC(hipMemcpyAsync(totals, B.totals.p, sizeof totals, hipMemcpyDeviceToHost, s->stream));
         HIPC(hipStreamSynchronize(s->stream));
+        if (use_graph && fp.tail_last == 0) {
+            // totals[3]: the latest bounce at which the tail took over in any batch (0xffff: not in every one).
+            const bool hint = totals[3] >= 1 && totals[3] < static_cast<unsigned long long>(P.max_bounces);
+            s->tail_hint = hint ? static_cast<uint32_t>(totals[3]) : 0u;
+            s->tail_hint_key = shape_key;
+        }
         if (stats) {
             stats->rays = totals[0] + totals[1];
             stats->primary_rays = totals[2];

@@ -109,6 +109,10 @@ struct pt_scene {
         ptd::FrameParams key{};
         int S = 0, trace_grid = 0, shade_grid = 0;
     } graph;
+    // The bounce at which the last graph render of shape tail_hint_key had
+    // the tail launch take over (0: it did not in every batch).
+    ptd::FrameParams tail_hint_key{};
+    uint32_t tail_hint = 0;
     void drop_graph() {
         if (graph.exec) (void)hipGraphExecDestroy(graph.exec);
         graph.exec = nullptr;

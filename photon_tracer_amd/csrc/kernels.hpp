@@ -97,6 +97,8 @@ struct FrameParams {
                                  // the diffuse hits the guide's inference evaluated
     uint32_t tail_threshold;  // queues below this many rays go to the tail launch (0: never)
     uint32_t camera_packets;  // 1: camera rays of a scene without instances are walked as wave packets
+    uint32_t tail_last;       // bounce at which the tail launch takes over whatever the queue sizes, and
+                              // the last bounce launched (0: every bounce launched, the tail by queue size)
     // The learned guide: 0 off, 1 diffuse bounces sample it (one-sample MIS
     // with the cosine lobe), 2 they also record training vertices.
     int guide_mode;

@@ -1362,6 +1362,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
 __device__ __forceinline__ bool tail_starts(const FrameParams& fp, int b) {
     if (fp.frame->tail_from < static_cast<uint32_t>(b)) return false;
     if (fp.guide_mode && b < fp.guide_bounces) return false;   // the tail shades without the guide
+    if (fp.tail_last != 0 && static_cast<uint32_t>(b) == fp.tail_last) return true;
     return fp.ctrl[b - 1].n_shadow + fp.ctrl[b].n_rays < fp.tail_threshold;
 }
 
@@ -1639,6 +1640,9 @@ __global__ void k_finish(FrameParams fp) {
     fp.totals[0] += rays;
     fp.totals[1] += shadow;
     fp.totals[2] += fp.ctrl[0].n_rays;
+    // The latest bounce at which the tail launch took over in any batch (0xffff: a batch without it).
+    const unsigned long long tf = fp.frame->tail_from == ~0u ? 0xffffull : fp.frame->tail_from;
+    fp.totals[3] = fp.totals[3] > tf ? fp.totals[3] : tf;
     for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
     fp.ctrl[MAX_BOUNCES] = Ctrl{};
     fp.frame->batch += 1;
@@ -1657,7 +1661,7 @@ __global__ __launch_bounds__(256) void k_clear_frame(FrameParams fp) {
     if (px == 0) {
         fp.frame->batch = 0;
         fp.frame->tail_from = ~0u;
-        fp.totals[0] = fp.totals[1] = fp.totals[2] = 0;
+        fp.totals[0] = fp.totals[1] = fp.totals[2] = fp.totals[3] = 0;
         for (int b = 0; b <= fp.depth && b < MAX_BOUNCES; ++b) fp.ctrl[b] = Ctrl{};
         fp.ctrl[MAX_BOUNCES] = Ctrl{};
     }
@@ -1698,6 +1702,8 @@ static void launch_loop(const FrameParams& fp, int trace_grid, int shade_grid, h
             k_tail_shadow<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
             if (guided) k_tail<S, INST, true><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
             else k_tail<S, INST><<<trace_grid, TRACE_BLOCK, 0, stream>>>(fp, b);
+            // The tail has every path from here on: no launch after it has work.
+            if (fp.tail_last != 0 && static_cast<uint32_t>(b) == fp.tail_last) break;
         }
         if (ev) (void)hipEventRecord(ev[2 * b], stream);
         if (b == 0) {
