@@ -212,10 +212,13 @@ __device__ __forceinline__ float vmf_pdf(const Lobe& l, f3 w) { return vmf_pdf(l
 // trainer's gradient keeps the correctly rounded library functions. The
 // sampler and the pdf share these functions and the lobe axis is
 // renormalised, so a sample is weighted by the density it was drawn from.
-__device__ __forceinline__ float fast_sigmoid(float v) { return __fdividef(1.0f, 1.0f + __expf(-v)); }
+// Quotients take the hardware reciprocal (within 1 ulp): __fdividef is the
+// full correctly rounded division sequence on this target, ten instructions.
+__device__ __forceinline__ float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fast_sigmoid(float v) { return __builtin_amdgcn_rcpf(1.0f + __expf(-v)); }
 
 __device__ __forceinline__ float fast_vmf_pdf(f3 mu, float kappa, f3 w) {
-    return __fdividef(kappa, 2 * PI_F * (1.0f - __expf(-2 * kappa))) * __expf(kappa * (dot(mu, w) - 1));
+    return fast_div(kappa, 2 * PI_F * (1.0f - __expf(-2 * kappa))) * __expf(kappa * (dot(mu, w) - 1));
 }
 
 // Lobe k's axis and concentration from the outputs o (FP32 or FP16).
@@ -256,19 +259,19 @@ __device__ __forceinline__ float mixture_pdf(const T* o, f3 w) {
         lobe_at(o, k, mu, kappa);
         p += __expf(static_cast<float>(o[4 * k + 3]) - mx) * fast_vmf_pdf(mu, kappa, w);
     }
-    return __fdividef(p, sum);
+    return fast_div(p, sum);
 }
 
 // A direction from lobe (nn, kappa): the inversion c = 1 + log(u1 + (1 - u1)
 // e^{-2 kappa}) / kappa for the cosine to the axis, a uniform azimuth u2.
 __device__ __forceinline__ f3 vmf_sample(f3 nn, float kappa, float u1, float u2) {
-    const float cw = fminf(1.0f, fmaxf(-1.0f, 1 + __fdividef(__logf(u1 + (1 - u1) * __expf(-2 * kappa)), kappa)));
+    const float cw = fminf(1.0f, fmaxf(-1.0f, 1 + fast_div(__logf(u1 + (1 - u1) * __expf(-2 * kappa)), kappa)));
     const float s = sqrtf(fmaxf(0.0f, 1 - cw * cw));
     const float phi = 2 * PI_F * u2;
     float sphi, cphi;
     __sincosf(phi, &sphi, &cphi);
     const float sign = copysignf(1.0f, nn.z);
-    const float pp = -1 / (sign + nn.z), qq = nn.x * nn.y * pp;
+    const float pp = -__builtin_amdgcn_rcpf(sign + nn.z), qq = nn.x * nn.y * pp;
     const f3 t1 = mk(1 + sign * nn.x * nn.x * pp, sign * qq, -sign * nn.x);
     const f3 t2 = mk(qq, sign + nn.y * nn.y * pp, -nn.y);
     return t1 * (s * cphi) + t2 * (s * sphi) + nn * cw;
@@ -346,7 +349,7 @@ __device__ __forceinline__ float decoded_pdf(const _Float16* q, f3 w) {
         decoded_lobe(q, k, mu, kappa, wk);
         p += wk * fast_vmf_pdf(mu, kappa, w);
     }
-    return __fdividef(p, decoded_weight_sum(q));
+    return fast_div(p, decoded_weight_sum(q));
 }
 
 __device__ __forceinline__ f3 decoded_sample(const _Float16* q, float u0, float u1, float u2) {

@@ -1145,7 +1145,7 @@ __device__ __forceinline__ void infer_staged(const FrameParams& fp, const guide:
             sum += e[g4];
         }
         sum += __shfl_xor(sum, 32, 64);
-        const float inv = __fdividef(1.0f, sum);
+        const float inv = __builtin_amdgcn_rcpf(sum);
         if (static_cast<uint32_t>(32 * nb + c) < count) {
             _Float16* out = fp.gout + static_cast<size_t>(slot[nb]) * guide::OUT + 4 * kh;
 #pragma unroll
