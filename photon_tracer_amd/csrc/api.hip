@@ -568,8 +568,8 @@ pt_status pt_render(pt_scene* s, const pt_render_params* prm, float* rgb, float*
             fp.gv = G->view();
             fp.gclamp = G->weight_clamp;
             fp.guide_bounces = G->bounces;
-            B.gout.ensure(paths * ptd::guide::OUT * 2);
-            fp.gout = B.gout.as<_Float16>();
+            B.gout.ensure(paths * ptd::guide::LOBES * 4);
+            fp.gout = B.gout.as<uint32_t>();
             B.gmis.ensure(paths * 4);
             fp.gmis = B.gmis.as<float>();
             if (training) {

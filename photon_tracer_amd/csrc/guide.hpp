@@ -304,43 +304,69 @@ __device__ __forceinline__ f3 mixture_sample(const T* o, float u0, float u1, flo
 
 // ---------------------------------------------------------------------------
 // The mixture as the path loop carries it from the inference stage to the
-// shade stage: per lobe four FP16 values (x, y, s, w) in place of the lobe's
-// four outputs -- the axis' first two components, kappa with the sign of
-// the axis' third component, and the lobe's softmax weight. The inference
-// kernel decodes each lobe once (its lanes hold whole lobes); the shade
-// stage's sampler and pdf then read the same rounded values, rebuild the
-// axis from them the same way, and weight by the density they sample, so the
-// estimator stays unbiased.
+// shade stage: one 32-bit word per lobe -- the axis in octahedral form with
+// 10 bits a coordinate (bits 0-19, within about 0.2 degrees), log kappa over
+// its clamp range in 6 bits (bits 20-25, kappa within 11%), and the lobe's
+// softmax weight relative to the mixture's largest in 6 bits (bits 26-31;
+// lobes under 1/126 of it are dropped). The inference kernel encodes each
+// lobe once (its lanes hold whole lobes); the shade stage's sampler and pdf
+// decode the same words the same way and weight by the density they sample,
+// so the estimator stays unbiased whatever the rounding.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void decode_lobe(float o0, float o1, float o2, float& x, float& y, float& s) {
+constexpr float MIX_KAPPA_STEP = (LOG_KAPPA_MAX - LOG_KAPPA_MIN) / 63.0f;
+
+// Octahedral fold of the lower hemisphere (z < 0) onto the square's corners.
+__device__ __forceinline__ void oct_fold(float& px, float& py) {
+    const float qx = copysignf(1.0f - fabsf(py), px), qy = copysignf(1.0f - fabsf(px), py);
+    px = qx;
+    py = qy;
+}
+
+// The word of a lobe from its outputs (theta logit, phi logit, log kappa)
+// and its weight relative to the largest lobe's, rel in (0, 1].
+__device__ __forceinline__ uint32_t encode_lobe(float o0, float o1, float o2, float rel) {
     // The angles in turns (theta = pi sigmoid, phi = 2 pi sigmoid), the unit
     // the hardware's sine and cosine take.
     const float th = 0.5f * fast_sigmoid(o0), ph = fast_sigmoid(o1);
     const float st = __builtin_amdgcn_sinf(th), ct = __builtin_amdgcn_cosf(th);
     const float sp = __builtin_amdgcn_sinf(ph), cp = __builtin_amdgcn_cosf(ph);
-    x = st * cp;
-    y = st * sp;
-    s = copysignf(__expf(fminf(fmaxf(o2, LOG_KAPPA_MIN), LOG_KAPPA_MAX)), ct);
+    const float x = st * cp, y = st * sp;
+    const float inv = __builtin_amdgcn_rcpf(fabsf(x) + fabsf(y) + fabsf(ct));
+    float px = x * inv, py = y * inv;
+    if (ct < 0) oct_fold(px, py);
+    const uint32_t ux = static_cast<uint32_t>(fminf(fmaxf(px * 511.5f + 512.0f, 0.0f), 1023.0f));
+    const uint32_t uy = static_cast<uint32_t>(fminf(fmaxf(py * 511.5f + 512.0f, 0.0f), 1023.0f));
+    const float lk = fminf(fmaxf(o2, LOG_KAPPA_MIN), LOG_KAPPA_MAX) - LOG_KAPPA_MIN;
+    const uint32_t uk = static_cast<uint32_t>(fminf(lk * (1.0f / MIX_KAPPA_STEP) + 0.5f, 63.0f));
+    const uint32_t uw = static_cast<uint32_t>(fminf(fmaxf(rel * 63.0f + 0.5f, 0.0f), 63.0f));
+    return ux | (uy << 10) | (uk << 20) | (uw << 26);
 }
 
-// Lobe k of a decoded mixture: its unit axis, kappa and weight.
-__device__ __forceinline__ void decoded_lobe(const _Float16* q, int k, f3& mu, float& kappa, float& w) {
-    const float x = static_cast<float>(q[4 * k]), y = static_cast<float>(q[4 * k + 1]);
-    const float s = static_cast<float>(q[4 * k + 2]);
-    w = static_cast<float>(q[4 * k + 3]);
-    const f3 m = mk(x, y, copysignf(sqrtf(fmaxf(0.0f, 1 - x * x - y * y)), s));
+__device__ __forceinline__ float decoded_weight(const uint32_t* q, int k) { return static_cast<float>(q[k] >> 26); }
+
+// Lobe k of an encoded mixture: its unit axis, kappa and (unnormalised) weight.
+// The octahedral point is never the origin (its coordinates are odd
+// multiples of 1/1023), so the axis always normalises.
+__device__ __forceinline__ void decoded_lobe(const uint32_t* q, int k, f3& mu, float& kappa, float& w) {
+    const uint32_t v = q[k];
+    float px = static_cast<float>(v & 1023u) * (2.0f / 1023.0f) - 1.0f;
+    float py = static_cast<float>((v >> 10) & 1023u) * (2.0f / 1023.0f) - 1.0f;
+    const float pz = 1.0f - fabsf(px) - fabsf(py);
+    if (pz < 0) oct_fold(px, py);
+    const f3 m = mk(px, py, pz);
     mu = m * rsqrtf(dot(m, m));
-    kappa = fabsf(s);
+    kappa = __expf(LOG_KAPPA_MIN + static_cast<float>((v >> 20) & 63u) * MIX_KAPPA_STEP);
+    w = static_cast<float>(v >> 26);
 }
 
-__device__ __forceinline__ float decoded_weight_sum(const _Float16* q) {
+__device__ __forceinline__ float decoded_weight_sum(const uint32_t* q) {
     float sum = 0;
 #pragma unroll
-    for (int k = 0; k < LOBES; ++k) sum += static_cast<float>(q[4 * k + 3]);
+    for (int k = 0; k < LOBES; ++k) sum += decoded_weight(q, k);
     return sum;
 }
 
-__device__ __forceinline__ float decoded_pdf(const _Float16* q, f3 w) {
+__device__ __forceinline__ float decoded_pdf(const uint32_t* q, f3 w) {
     float p = 0;
 #pragma unroll 1
     for (int k = 0; k < LOBES; ++k) {
@@ -352,14 +378,14 @@ __device__ __forceinline__ float decoded_pdf(const _Float16* q, f3 w) {
     return fast_div(p, decoded_weight_sum(q));
 }
 
-__device__ __forceinline__ f3 decoded_sample(const _Float16* q, float u0, float u1, float u2) {
+__device__ __forceinline__ f3 decoded_sample(const uint32_t* q, float u0, float u1, float u2) {
     const float target = u0 * decoded_weight_sum(q);
     float acc = 0;
     int sel = LOBES - 1;
     bool chosen = false;
 #pragma unroll
     for (int k = 0; k < LOBES; ++k) {
-        acc += static_cast<float>(q[4 * k + 3]);
+        acc += decoded_weight(q, k);
         if (!chosen && target < acc) {
             sel = k;
             chosen = true;

@@ -103,7 +103,7 @@ struct FrameParams {
     // with the cosine lobe), 2 they also record training vertices.
     int guide_mode;
     guide::GuideView gv;
-    _Float16* gout;           // per slot of the current bounce's queue: the decoded mixture, 32 FP16 values
+    uint32_t* gout;           // per slot of the current bounce's queue: the encoded mixture, a word per lobe
     float4* grec;             // GUIDE_RECORDS per path, 4 float4 each (layout at record_vertex)
     float* gwt;               // per record: its training weight
     uint32_t* gblk;           // per block of 256 records: count of weighted records, then its offset

@@ -916,7 +916,7 @@ __device__ __forceinline__ void record_vertex(const FrameParams& fp, uint32_t p,
 // this hit when diffuse bounces sample the guide, else null.
 template <bool GUIDED = false>
 __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, uint32_t p, const Hit& h, f3 o, f3 d,
-                                                 PathState& ps, const _Float16* go, const LightView& lv) {
+                                                 PathState& ps, const uint32_t* go, const LightView& lv) {
     const SceneView& sc = fp.sc;
     ShadeResult r;
     r.cont = r.shadow = r.backface = false;
@@ -1092,8 +1092,8 @@ __device__ __forceinline__ ShadeResult shade_hit(const FrameParams& fp, int b, u
 
 // The guide's mixture for the shade stage of bounce b: for every queued hit
 // on a diffuse (Lambertian) surface, the network at its point and normal,
-// decoded to 32 FP16 values at gout[slot] (per lobe: axis x, y, kappa with
-// the axis' z sign, weight). The block first copies the network's weights
+// encoded as one 32-bit word per lobe at gout[slot] (octahedral axis,
+// log kappa, weight relative to the largest lobe's). The block first copies the network's weights
 // into LDS. Each wave then walks its runs of 64 slots and stages the
 // diffuse hits among them (their five encoding scalars and slot) in its own
 // LDS list; whenever 64 are staged the network runs on them, in registers
@@ -1130,32 +1130,21 @@ __device__ __forceinline__ void infer_staged(const FrameParams& fp, const guide:
     guide::f16v O[2];
     guide::forward_regs(guide::LdsWeights{net}, X, O);
     // The lane holds whole lobes (2 g4 + kh, g4 < 4) of its two entries: each
-    // is decoded once here for the shade stage, the softmax taking the other
-    // half's four lobes from the partner lane.
+    // is encoded once here for the shade stage, its weight relative to the
+    // largest of the eight, the other half's four lobes taken from the
+    // partner lane.
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
         float mx = O[nb][3];
 #pragma unroll
         for (int g4 = 1; g4 < 4; ++g4) mx = fmaxf(mx, O[nb][4 * g4 + 3]);
         mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-        float e[4], sum = 0;
-#pragma unroll
-        for (int g4 = 0; g4 < 4; ++g4) {
-            e[g4] = __expf(O[nb][4 * g4 + 3] - mx);
-            sum += e[g4];
-        }
-        sum += __shfl_xor(sum, 32, 64);
-        const float inv = __builtin_amdgcn_rcpf(sum);
         if (static_cast<uint32_t>(32 * nb + c) < count) {
-            _Float16* out = fp.gout + static_cast<size_t>(slot[nb]) * guide::OUT + 4 * kh;
+            uint32_t* out = fp.gout + static_cast<size_t>(slot[nb]) * guide::LOBES + kh;
 #pragma unroll
-            for (int g4 = 0; g4 < 4; ++g4) {
-                float x, y, sk;
-                guide::decode_lobe(O[nb][4 * g4], O[nb][4 * g4 + 1], O[nb][4 * g4 + 2], x, y, sk);
-                const guide::h4 hv{static_cast<_Float16>(x), static_cast<_Float16>(y), static_cast<_Float16>(sk),
-                                   static_cast<_Float16>(e[g4] * inv)};
-                *reinterpret_cast<guide::h4*>(out + 8 * g4) = hv;
-            }
+            for (int g4 = 0; g4 < 4; ++g4)
+                out[2 * g4] = guide::encode_lobe(O[nb][4 * g4], O[nb][4 * g4 + 1], O[nb][4 * g4 + 2],
+                                                 __expf(O[nb][4 * g4 + 3] - mx));
         }
     }
 }
@@ -1251,11 +1240,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
     if (PRIMARY) fc = *fp.frame;
     const int cur = b & 1, nxt = cur ^ 1;
     __shared__ BlockAppend<4> sh;
-    // GUIDED: each thread's decoded mixture (32 FP16 values), copied from HBM with four
-    // 16-byte loads into an LDS row of 17 words (an odd stride: the lobe loops'
-    // reads, one output at a time, fall in distinct banks), instead of
-    // gathering them two bytes at a time from HBM inside the lobe loops.
-    constexpr int GROW = guide::OUT / 2 + 1;
+    // GUIDED: each thread's encoded mixture (eight 32-bit words), copied from
+    // HBM with two 16-byte loads into an LDS row of 9 words (an odd stride:
+    // the lobe loops' reads, one word at a time, fall in distinct banks),
+    // instead of gathered from HBM inside the lobe loops.
+    constexpr int GROW = guide::LOBES + 1;
     __shared__ uint32_t gtile[GUIDED ? SHADE_BLOCK * GROW : 1];
     // Light samples are appended in two runs per block, those to even- and
     // to odd-numbered lights, so a traversal wave mostly holds rays toward
@@ -1270,19 +1259,19 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(FrameParams fp, int b) {
         r.cont = r.shadow = r.backface = false;
         uint32_t p = 0;
         if (q < n) {
-            const _Float16* go = nullptr;
+            const uint32_t* go = nullptr;
             if (GUIDED && b < fp.guide_bounces) {
-                const uint4* src = reinterpret_cast<const uint4*>(fp.gout + static_cast<size_t>(q) * guide::OUT);
+                const uint4* src = reinterpret_cast<const uint4*>(fp.gout + static_cast<size_t>(q) * guide::LOBES);
                 uint32_t* row = gtile + threadIdx.x * GROW;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < 2; ++i) {
                     const uint4 v = src[i];
                     row[4 * i] = v.x;
                     row[4 * i + 1] = v.y;
                     row[4 * i + 2] = v.z;
                     row[4 * i + 3] = v.w;
                 }
-                go = reinterpret_cast<const _Float16*>(row);
+                go = row;
             }
             const Hit h = fp.hits[q];
             f3 o, d;
